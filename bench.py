@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 sync-free SpTRSV on one MI355X.
+
+Default workload: FP64 forward solve of the lower triangle (incl. diagonal) of
+the 3D 7-point Laplacian on a 128^3 grid (n = 2,097,152, nnz = 8,339,456,
+382 DAG levels), diag 6, off-diagonal -1, x ~ U(-1, 1) from
+numpy default_rng(0), b = L x, one right-hand side.  A step is one solve of
+that system on inputs already resident in HBM; W untimed solves, then K timed
+solves between two stream synchronisations.
+
+Prints one JSON line.  `roofline.achieved` is the algorithmic bytes of one
+solve (CSC values + row indices + column pointers read once, b read once,
+x written once) divided by the mean executor-kernel duration measured with HIP
+events on the stream the kernel runs on; `roofline.traffic` is the HBM traffic
+per launch measured with rocprofv3 counters, read from
+profiles/traffic_<workload>.json when that file exists.  `cpu_baseline` times
+the reference's own serial executor (compiled from the reference sources into
+oracle/_ref) or, without it, the oracle restatement, single-threaded, on the
+same matrix.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "benchmark-sptrsv_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "solves/sec + achieved HBM GB/s vs FP64 sync-free at rel err ≤1e-10"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from sptrsv_amd import matrices as mx
+    if name == "lap3d":
+        f = mx.laplacian3d_lower(128, 128, 128)
+        return f, mx.x_uniform(f.n, seed=0), 382
+    if name == "lap2d":
+        f = mx.laplacian2d_lower(256, 256)
+        return f, np.ones(f.n), 511
+    if name == "powerlaw":
+        f = mx.power_law_lower(1_000_000, seed=0)
+        return f, mx.x_uniform(f.n, seed=1), None
+    if name == "bidiag":
+        f = mx.bidiagonal(1_000_000)
+        return f, mx.x_uniform(f.n, seed=0), 1_000_000
+    if name == "batch":
+        f = mx.laplacian2d_batch(512, seed=0)
+        return f, mx.x_uniform(f.n, seed=0), None
+    raise SystemExit(f"unknown workload {name}")
+
+
+WORKLOAD_DESC = {
+    "lap3d": "lower triangle of 3D 7-point Laplacian 128^3 (n=2,097,152), diag 6, off -1",
+    "lap2d": "lower triangle of 2D 5-point Laplacian 256^2 (n=65,536), diag 4, off -1",
+    "powerlaw": "random power-law lower triangular n=1,000,000, Zipf(2) cap 1000, window 50,000",
+    "bidiag": "bidiagonal chain n=1,000,000, diag 2, sub -1",
+    "batch": "512 uneven 2D 5-point Laplacian lower factors (sides log-U[32,256]) in one launch",
+}
+
+
+def cpu_baseline(f, b, budget_s):
+    """Single-threaded serial sync-free executor on the same matrix."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import sptrsv_oracle as orc
+    kind = "reference" if orc.ref(np.float64) is not None else "port"
+    fn = orc.ref_solve if kind == "reference" else orc.solve
+    indeg = orc.analyser(f.rowIdx, f.n)  # analysis is outside the timed executor
+    colPtr, rowIdx = f.colPtr, f.rowIdx
+    reps, t0 = 0, time.perf_counter()
+    times = []
+    while time.perf_counter() - t0 < budget_s or reps < 2:
+        t = time.perf_counter()
+        x_cpu = fn(colPtr, rowIdx, f.val, b, in_degree=indeg)
+        times.append(time.perf_counter() - t)
+        reps += 1
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 3), "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": f"{reps} serial FP64 executor runs on the same matrix and b "
+                      f"(median {med * 1e3:.2f} ms each, analysis excluded), "
+                      f"{os.cpu_count()} host cores visible"}, x_cpu
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="lap3d", choices=sorted(WORKLOAD_DESC))
+    ap.add_argument("--rhs", type=int, default=1)
+    ap.add_argument("--form", default="plain", choices=["plain", "ir"])
+    ap.add_argument("--ir-iters", type=int, default=2)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the solver has no multi-GPU path "
+              "(a single triangular system is one GPU's work; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import sptrsv_amd as S
+
+    if S.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    log(f"device: {S.device_name()}")
+    t = time.perf_counter()
+    f, x_true, levels = make_workload(args.workload)
+    rhs = args.rhs
+    if rhs > 1:
+        from sptrsv_amd import matrices as mx
+        x_true = mx.x_normal(f.n, rhs, seed=1)
+    b = f.spmv(x_true)
+    log(f"workload {args.workload}: n={f.n} nnz={f.nnz} rhs={rhs} "
+        f"(generated in {time.perf_counter() - t:.1f} s)")
+
+    flags = S.PLAN_VALIDATE | (S.PLAN_MIXED if args.form == "ir" else 0)
+    dP = S.DeviceArray.from_host(f.colPtr)
+    dI = S.DeviceArray.from_host(f.rowIdx)
+    dV = S.DeviceArray.from_host(f.val)
+    plan = S.Plan(dP, dI, dV, f.n, f.nnz, flags=flags, sys_ptr=f.sys_ptr)
+    db = S.DeviceArray.from_host(b)
+    dx = S.DeviceArray(b.shape, np.float64)
+    stream = S.Stream()
+    tuned = None
+    if not args.no_tune:
+        tuned, tms = plan.autotune(rhs=rhs, mode=1 if args.form == "ir" else 0)
+        log(f"autotuned config {tuned} ({tms:.3f} ms)")
+
+    def step():
+        if args.form == "ir":
+            plan.solve_ir(db, dx, rhs=rhs, iters=args.ir_iters, stream=stream)
+        else:
+            plan.solve(db, dx, rhs=rhs, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    assert plan.check(stream) == 0, "solve timed out"
+
+    # timed region: K steps bracketed by stream synchronisation, HIP events
+    # around each launch on the solve stream
+    evs = [(S.Event(), S.Event()) for _ in range(args.steps)]
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    st = plan.check(stream)
+    assert st == 0, f"solve status {st}"
+    per = np.array([e0.elapsed_ms(e1) for e0, e1 in evs])
+    ms_step = wall * 1e3 / args.steps
+    x = dx.to_host()
+
+    # accuracy against the known solution (b was built as L x_true)
+    d = np.abs(x - x_true.reshape(x.shape))
+    err_true = {"l1": float(d.sum() / np.abs(x_true).sum()),
+                "linf": float(d.max() / np.abs(x_true).max())}
+
+    solve_bytes = f.algorithmic_bytes(rhs=rhs, value_bytes=8)
+    if args.form == "ir":
+        it = args.ir_iters
+        s32 = f.algorithmic_bytes(rhs=rhs, value_bytes=4)
+        resid = f.nnz * 12 + (f.n + 1) * 4 + 3 * f.n * rhs * 8 + f.n * rhs * 4
+        solve_bytes = (it + 1) * s32 + it * resid + f.n * rhs * (8 + 4) + f.n * rhs * 12
+    kern_ms = float(np.mean(per))
+    achieved = solve_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.form}_rhs{rhs}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    pinfo = plan.info()
+    out = {
+        "metric": METRIC,
+        "value": round(rhs * args.steps / wall, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.form == "plain" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": WORKLOAD_DESC[args.workload] + f", nrhs={rhs}, "
+                        + ("FP64 sync-free" if args.form == "plain"
+                           else f"FP32 solve + {args.ir_iters} FP64 refinement steps"),
+            "n": f.n, "nnz": f.nnz, "nrhs": rhs, "dag_levels": levels,
+            "executor": tuned or plan.get_config(rhs),
+            "analysis_ms": round(pinfo["analysis_ms"], 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "algorithmic_bytes": solve_bytes,
+            "kernel_ms_mean": round(kern_ms, 5),
+            "kernel_ms_min": round(float(per.min()), 5),
+        },
+        "rel_err_vs_x_true": err_true,
+    }
+    if not args.no_cpu and rhs == 1:
+        cpu, x_cpu = cpu_baseline(f, b.reshape(-1), args.cpu_seconds)
+        out["cpu_baseline"] = cpu
+        # the CPU run is also the checker: same matrix, same b
+        d = np.abs(x.reshape(-1) - x_cpu)
+        l1 = float(d.sum() / np.abs(x_cpu).sum())
+        linf = float(d.max() / np.abs(x_cpu).max())
+        tol = 1e-12 if args.form == "plain" else 1e-10
+        out["rel_err_vs_cpu_serial"] = {"l1": l1, "linf": linf, "tol": tol,
+                                        "pass": bool(l1 <= tol and linf <= tol)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1555 @@
+// sptrsv_api.hip -- C ABI of the gfx950 sync-free SpTRSV library
+// (declarations and reference citations in include/sptrsv_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sptrsv_hip.h"
+#include "sptrsv_exec.hpp"
+#include "sptrsv_ir.hpp"
+#include "sptrsv_transpose.hpp"
+
+using namespace sptrsv;
+
+namespace {
+
+int g_verbose = 0;
+int g_bench_repeat = 100;  // BENCH_REPEAT, SpTRSV_cuda/common.h:13-14
+// polls of one spin-wait before a solve gives up: seconds of polling, far
+// beyond any legitimate wait of a valid factor
+constexpr long long kDefaultSpinCap = 1ll << 24;
+
+#define HIP_TRY(expr)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (expr);                                               \
+    if (e_ != hipSuccess) {                                               \
+      if (g_verbose)                                                      \
+        fprintf(stderr, "sptrsv: %s failed: %s (%s:%d)\n", #expr,         \
+                hipGetErrorString(e_), __FILE__, __LINE__);               \
+      return e_ == hipErrorOutOfMemory ? SPTRSV_ERR_NO_MEMORY             \
+                                       : SPTRSV_ERR_HIP;                  \
+    }                                                                     \
+  } while (0)
+
+#define ST_TRY(expr)             \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ != SPTRSV_OK) return s_; \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int grid_for(size_t work, int block) {
+  size_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+template <typename T>
+int fill_async(T *p, size_t len, T v, hipStream_t s) {
+  if (len == 0) return SPTRSV_OK;
+  k_fill<T><<<grid_for(len, 256), 256, 0, s>>>(p, len, v);
+  HIP_TRY(hipGetLastError());
+  return SPTRSV_OK;
+}
+
+// Device buffer owned by the library.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want, bool zero) {
+    if (want <= bytes) return SPTRSV_OK;
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    HIP_TRY(hipMalloc(&p, want));
+    bytes = want;
+    if (zero) HIP_TRY(hipMemset(p, 0, want));
+    return SPTRSV_OK;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// ------------------------------------------------------------------ scan --
+int scan_rec(int *d, int len, hipStream_t s) {
+  if (len <= 1) {
+    if (len == 1) HIP_TRY(hipMemsetAsync(d, 0, sizeof(int), s));
+    return SPTRSV_OK;
+  }
+  const int nb = (len + kScanTile - 1) / kScanTile;
+  DevBuf sums;
+  ST_TRY(sums.ensure(sizeof(int) * (size_t)nb, false));
+  k_scan_tile<<<nb, kScanThreads, 0, s>>>(d, len, sums.as<int>());
+  int st = SPTRSV_OK;
+  if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK && nb > 1) {
+    st = scan_rec(sums.as<int>(), nb, s);
+    if (st == SPTRSV_OK) {
+      k_scan_add<<<nb, kScanThreads, 0, s>>>(d, len, sums.as<int>());
+      if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
+    }
+  }
+  // the temporary must outlive the kernels that read it
+  if (hipStreamSynchronize(s) != hipSuccess) st = SPTRSV_ERR_HIP;
+  sums.release();
+  return st;
+}
+
+// ------------------------------------------------------------- transpose --
+int check_csr(int m, int n, int nnz, const int *rowPtr, const int *colIdx,
+              hipStream_t s) {
+  int ends[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(&ends[0], rowPtr, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&ends[1], rowPtr + m, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (ends[0] != 0 || ends[1] != nnz) return SPTRSV_ERR_INDEX;
+  DevBuf flags;
+  ST_TRY(flags.ensure(sizeof(int), true));
+  k_check_csr<<<grid_for(std::max(m, nnz), 256), 256, 0, s>>>(rowPtr, colIdx, m,
+                                                              n, nnz, flags.as<int>());
+  int f = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&f, flags.p, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  flags.release();
+  if (e != hipSuccess) return SPTRSV_ERR_HIP;
+  return f ? SPTRSV_ERR_INDEX : SPTRSV_OK;
+}
+
+int csr2csc_impl(int m, int n, int nnz, const int *rowPtr, const int *colIdx,
+                 const void *val, int vt, int *colPtr, int *rowIdx,
+                 void *outVal, hipStream_t s) {
+  if (m > 0) ST_TRY(check_csr(m, n, nnz, rowPtr, colIdx, s));
+  HIP_TRY(hipMemsetAsync(colPtr, 0, sizeof(int) * ((size_t)n + 1), s));
+  if (nnz > 0) {
+    k_histogram<<<grid_for(nnz, 256), 256, 0, s>>>(colIdx, nnz, colPtr);
+    HIP_TRY(hipGetLastError());
+  }
+  ST_TRY(scan_rec(colPtr, n + 1, s));
+  if (nnz == 0) return SPTRSV_OK;
+
+  DevBuf cursor, keys, tmp, segs, longs, counts;
+  int st = SPTRSV_OK;
+  auto fail = [&](int code) {
+    st = code;
+    return code;
+  };
+  do {
+    if (cursor.ensure(sizeof(int) * ((size_t)n + 1), false) ||
+        keys.ensure(sizeof(unsigned long long) * (size_t)nnz, false) ||
+        segs.ensure(sizeof(int) * 2 * ((size_t)n + 1), false) ||
+        longs.ensure(sizeof(int) * 2 * ((size_t)n + 1), false) ||
+        counts.ensure(sizeof(int) * 2, true)) {
+      fail(SPTRSV_ERR_NO_MEMORY);
+      break;
+    }
+    if (hipMemcpyAsync(cursor.p, colPtr, sizeof(int) * ((size_t)n + 1),
+                       hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      fail(SPTRSV_ERR_HIP);
+      break;
+    }
+    k_scatter_keys<<<grid_for(nnz, 256), 256, 0, s>>>(
+        rowPtr, colIdx, m, nnz, cursor.as<int>(),
+        keys.as<unsigned long long>());
+    int *seg_start = segs.as<int>(), *seg_len = seg_start + (n + 1);
+    int *long_start = longs.as<int>(), *long_len = long_start + (n + 1);
+    k_sort_small<<<grid_for(n, 256), 256, 0, s>>>(
+        colPtr, n, keys.as<unsigned long long>(), seg_start, seg_len,
+        counts.as<int>(), long_start, long_len, counts.as<int>() + 1);
+    if (hipGetLastError() != hipSuccess) {
+      fail(SPTRSV_ERR_HIP);
+      break;
+    }
+    int hc[2] = {0, 0};
+    if (hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      fail(SPTRSV_ERR_HIP);
+      break;
+    }
+    if (hc[0] > 0) {
+      k_sort_lds<<<hc[0], kLdsThreads, 0, s>>>(keys.as<unsigned long long>(),
+                                               seg_start, seg_len);
+      if (hipGetLastError() != hipSuccess) {
+        fail(SPTRSV_ERR_HIP);
+        break;
+      }
+    }
+    if (hc[1] > 0) {
+      // long buckets: sort chunks of kLdsSeg in LDS, then merge pairwise
+      std::vector<int> ls(hc[1]), ll(hc[1]);
+      if (hipMemcpy(ls.data(), long_start, sizeof(int) * hc[1],
+                    hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(ll.data(), long_len, sizeof(int) * hc[1],
+                    hipMemcpyDeviceToHost) != hipSuccess) {
+        fail(SPTRSV_ERR_HIP);
+        break;
+      }
+      int maxlen = 0;
+      for (int v : ll) maxlen = std::max(maxlen, v);
+      if (tmp.ensure(sizeof(unsigned long long) * (size_t)maxlen, false)) {
+        fail(SPTRSV_ERR_NO_MEMORY);
+        break;
+      }
+      for (int b = 0; b < hc[1] && st == SPTRSV_OK; b++) {
+        const int start = ls[b], len = ll[b];
+        const int nch = (len + kLdsSeg - 1) / kLdsSeg;
+        std::vector<int> cs(nch), cl(nch);
+        for (int i = 0; i < nch; i++) {
+          cs[i] = start + i * kLdsSeg;
+          cl[i] = std::min(kLdsSeg, len - i * kLdsSeg);
+        }
+        DevBuf chunk;
+        if (chunk.ensure(sizeof(int) * 2 * (size_t)nch, false)) {
+          fail(SPTRSV_ERR_NO_MEMORY);
+          break;
+        }
+        int *dcs = chunk.as<int>(), *dcl = dcs + nch;
+        if (hipMemcpy(dcs, cs.data(), sizeof(int) * nch,
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dcl, cl.data(), sizeof(int) * nch,
+                      hipMemcpyHostToDevice) != hipSuccess) {
+          fail(SPTRSV_ERR_HIP);
+          break;
+        }
+        k_sort_lds<<<nch, kLdsThreads, 0, s>>>(keys.as<unsigned long long>(),
+                                               dcs, dcl);
+        unsigned long long *src = keys.as<unsigned long long>() + start;
+        unsigned long long *dst = tmp.as<unsigned long long>();
+        bool in_keys = true;
+        for (int w = kLdsSeg; w < len; w <<= 1) {
+          k_merge_pass<<<grid_for(len, 256), 256, 0, s>>>(src, dst, len, w);
+          std::swap(src, dst);
+          in_keys = !in_keys;
+        }
+        if (!in_keys &&
+            hipMemcpyAsync(keys.as<unsigned long long>() + start, src,
+                           sizeof(unsigned long long) * (size_t)len,
+                           hipMemcpyDeviceToDevice, s) != hipSuccess) {
+          fail(SPTRSV_ERR_HIP);
+          break;
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) fail(SPTRSV_ERR_HIP);
+        chunk.release();
+      }
+      if (st != SPTRSV_OK) break;
+    }
+    if (vt == SPTRSV_F64)
+      k_gather<double><<<grid_for(nnz, 256), 256, 0, s>>>(
+          keys.as<unsigned long long>(), nnz, (const double *)val, rowIdx,
+          (double *)(val ? outVal : nullptr));
+    else
+      k_gather<float><<<grid_for(nnz, 256), 256, 0, s>>>(
+          keys.as<unsigned long long>(), nnz, (const float *)val, rowIdx,
+          (float *)(val ? outVal : nullptr));
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+      fail(SPTRSV_ERR_HIP);
+  } while (0);
+  if (st != SPTRSV_OK) hipStreamSynchronize(s);
+  cursor.release();
+  keys.release();
+  tmp.release();
+  segs.release();
+  longs.release();
+  counts.release();
+  return st;
+}
+
+int validate_impl(const int *colPtr, const int *rowIdx, const void *val,
+                  int vt, int n, int nnz, int subst, hipStream_t s) {
+  int ends[2] = {0, 0};
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(&ends[0], colPtr, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&ends[1], colPtr + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (ends[0] != 0 || ends[1] != nnz) return SPTRSV_ERR_INDEX;
+  }
+  if (n == 0) return SPTRSV_OK;
+  DevBuf flags;
+  ST_TRY(flags.ensure(sizeof(int), true));
+  if (vt == SPTRSV_F64)
+    k_validate<double><<<grid_for(n, 256), 256, 0, s>>>(
+        colPtr, rowIdx, (const double *)val, n, nnz, subst, flags.as<int>());
+  else
+    k_validate<float><<<grid_for(n, 256), 256, 0, s>>>(
+        colPtr, rowIdx, (const float *)val, n, nnz, subst, flags.as<int>());
+  int f = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&f, flags.p, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  flags.release();
+  if (e != hipSuccess) return SPTRSV_ERR_HIP;
+  if (f & kBadIndex) return SPTRSV_ERR_INDEX;
+  if (f & kBadUnsorted) return SPTRSV_ERR_UNSORTED;
+  if (f & kBadTriangle) return SPTRSV_ERR_NOT_TRIANGULAR;
+  if (f & kBadMissingDiag) return SPTRSV_ERR_MISSING_DIAG;
+  if (f & kBadZeroDiag) return SPTRSV_ERR_ZERO_DIAG;
+  return SPTRSV_OK;
+}
+
+// ------------------------------------------------------ kernel dispatch ---
+template <typename T>
+using ExecKernel = void (*)(ExecArgs<T>);
+
+template <typename T, bool BWD>
+ExecKernel<T> pick_exec(int G, int RL, int maxrpl) {
+#define SPTRSV_CASE(g, rl, m) \
+  if (G == g && RL == rl && maxrpl == m) return k_exec<T, g, rl, m, BWD>;
+  SPTRSV_CASE(1, 1, 1)
+  SPTRSV_CASE(2, 1, 1)
+  SPTRSV_CASE(4, 1, 1)
+  SPTRSV_CASE(8, 1, 1)
+  SPTRSV_CASE(16, 1, 1)
+  SPTRSV_CASE(32, 1, 1)
+  SPTRSV_CASE(64, 1, 1)
+  SPTRSV_CASE(2, 2, 1)
+  SPTRSV_CASE(4, 4, 1)
+  SPTRSV_CASE(8, 8, 1)
+  SPTRSV_CASE(16, 16, 1)
+  SPTRSV_CASE(32, 32, 1)
+  SPTRSV_CASE(64, 64, 1)
+  SPTRSV_CASE(1, 1, 4)
+  SPTRSV_CASE(4, 1, 4)
+  SPTRSV_CASE(16, 1, 4)
+  SPTRSV_CASE(64, 1, 4)
+  SPTRSV_CASE(4, 4, 4)
+  SPTRSV_CASE(8, 8, 4)
+  SPTRSV_CASE(16, 16, 4)
+  SPTRSV_CASE(32, 32, 4)
+  SPTRSV_CASE(64, 64, 4)
+#undef SPTRSV_CASE
+  return nullptr;
+}
+
+template <typename T>
+ExecKernel<T> pick_exec_dir(bool bwd, int G, int RL, int maxrpl) {
+  return bwd ? pick_exec<T, true>(G, RL, maxrpl)
+             : pick_exec<T, false>(G, RL, maxrpl);
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int device_props(int *cus, size_t *lds_max) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int l = 0;
+  HIP_TRY(hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  *lds_max = (size_t)l;
+  return SPTRSV_OK;
+}
+
+}  // namespace
+
+// ===================================================================== plan
+struct WindowSet {
+  DevBuf winPtr;  // int[nwin+1]
+  DevBuf need;    // int2[n]
+  int nwin = 0;
+  int maxw = 0;
+};
+
+struct sptrsv_plan_s {
+  int n = 0, nnz = 0, subst = 0, vt = SPTRSV_F64, flags = 0, nsys = 1;
+  const int *colPtr = nullptr;
+  const int *rowIdx = nullptr;
+  const void *val = nullptr;
+  DevBuf inDegree;
+  DevBuf sync;  // int[4]: ticket, finished, err, pad
+  DevBuf gsum, gcnt;
+  std::map<long long, WindowSet *> windows;     // key: window*2 + partition
+  std::map<long long, sptrsv_config_t> configs;  // key: rhs*2 + is_f32_solve
+  // mixed precision
+  DevBuf val32, csrPtr, csrIdx, csrVal, xtmp, r32, d32;
+  long long spin_cap = kDefaultSpinCap;
+  int cus = 0;
+  size_t lds_max = 65536;
+  double analysis_ms = 0.0;
+  std::vector<int> sys_ptr;
+  double avg_offdiag = 0.0;
+};
+
+namespace {
+
+sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
+  sptrsv_config_t c;
+  memset(&c, 0, sizeof(c));
+  if (rhs <= 1) {
+    c.lanes_per_column = 64;  // one wavefront per column, as in the reference
+    c.rhs_lanes = 1;
+    c.rhs_per_lane = 1;
+    c.block = 1024;
+    c.window = 16;
+  } else {
+    int rl = 1;
+    while (rl < rhs && rl < 64) rl <<= 1;
+    c.lanes_per_column = rl;
+    c.rhs_lanes = rl;
+    c.rhs_per_lane = rhs > 64 ? std::min(4, (rhs + 63) / 64) : 1;
+    c.block = 256;
+    c.window = std::max(1, 256 / rl);
+  }
+  c.partition = SPTRSV_COL_PARTITION;
+  c.grid = 0;
+  (void)p;
+  return c;
+}
+
+int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
+  if (!is_pow2(c.lanes_per_column) || c.lanes_per_column > 64)
+    return SPTRSV_ERR_INVALID_ARG;
+  if (!(c.rhs_lanes == 1 || c.rhs_lanes == c.lanes_per_column))
+    return SPTRSV_ERR_INVALID_ARG;
+  if (c.rhs_per_lane != 1 && c.rhs_per_lane != 2 && c.rhs_per_lane != 3 &&
+      c.rhs_per_lane != 4)
+    return SPTRSV_ERR_INVALID_ARG;
+  if (c.block < 64 || c.block > 1024 || c.block % 64 != 0)
+    return SPTRSV_ERR_INVALID_ARG;
+  if (c.window < 1) return SPTRSV_ERR_INVALID_ARG;
+  if (c.partition != SPTRSV_COL_PARTITION &&
+      c.partition != SPTRSV_NNZ_PARTITION)
+    return SPTRSV_ERR_INVALID_ARG;
+  const size_t R = (size_t)c.rhs_lanes * c.rhs_per_lane;
+  const size_t lds = (size_t)c.window * (R * tsize + sizeof(int));
+  if (lds > p->lds_max) return SPTRSV_ERR_INVALID_ARG;
+  const int maxrpl = c.rhs_per_lane == 1 ? 1 : 4;
+  const bool ok = tsize == 8 ? pick_exec<double, false>(c.lanes_per_column,
+                                                       c.rhs_lanes, maxrpl) != nullptr
+                             : pick_exec<float, false>(c.lanes_per_column,
+                                                      c.rhs_lanes, maxrpl) != nullptr;
+  return ok ? SPTRSV_OK : SPTRSV_ERR_INVALID_ARG;
+}
+
+// Host restatement of the partitioners (utils.h:7-17, :19-67).
+int nnz_parts_needed(const std::vector<int> &len, int cap) {
+  int sum = 0, need = 0;
+  for (int v : len) {
+    if (sum + v > cap) {
+      sum = v;
+      need++;
+    } else {
+      sum += v;
+    }
+  }
+  return need;
+}
+
+void equal_nnz_partition_host(int nparts, int ncols, int nnz, const int *ptr,
+                              int *counts, int *displs) {
+  std::vector<int> len(ncols);
+  for (int c = 0; c < ncols; c++) len[c] = ptr[c + 1] - ptr[c];
+  int lo = 0, hi = nnz;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (nnz_parts_needed(len, mid) < nparts)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  int sum = 0, part = 0;
+  for (int c = 0; c < ncols; c++) {
+    if (sum + len[c] > lo) {
+      sum = len[c];
+      part++;
+      if (part >= nparts) break;  // cannot happen for a converged search
+      counts[part] = 1;
+      displs[part] = c;
+    } else {
+      sum += len[c];
+      counts[part]++;
+    }
+  }
+}
+
+int get_windows(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet **out,
+                hipStream_t s) {
+  const long long key = (long long)c.window * 2 + c.partition;
+  auto it = p->windows.find(key);
+  if (it != p->windows.end()) {
+    *out = it->second;
+    return SPTRSV_OK;
+  }
+  WindowSet *ws = new WindowSet();
+  const int n = p->n;
+  int st = SPTRSV_OK;
+  if (c.partition == SPTRSV_COL_PARTITION || n == 0) {
+    ws->nwin = n == 0 ? 0 : (n + c.window - 1) / c.window;
+    ws->maxw = std::min(c.window, std::max(n, 1));
+    st = ws->winPtr.ensure(sizeof(int) * ((size_t)ws->nwin + 1), false);
+    if (st == SPTRSV_OK) {
+      k_uniform_windows<<<grid_for(ws->nwin + 1, 256), 256, 0, s>>>(
+          ws->winPtr.as<int>(), ws->nwin, c.window, n);
+    }
+  } else {
+    // nnz-balanced windows over columns in solve order
+    std::vector<int> ptr(n + 1);
+    if (hipMemcpy(ptr.data(), p->colPtr, sizeof(int) * (n + 1),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      st = SPTRSV_ERR_HIP;
+    if (st == SPTRSV_OK) {
+      std::vector<int> sp(n + 1);  // column lengths in solve order
+      sp[0] = 0;
+      for (int q = 0; q < n; q++) {
+        const int col = p->subst == SPTRSV_SUBSTITUTION_BACKWARD ? n - 1 - q : q;
+        sp[q + 1] = sp[q] + (ptr[col + 1] - ptr[col]);
+      }
+      const int nparts = (n + c.window - 1) / c.window;
+      std::vector<int> counts(nparts, 0), displs(nparts, 0);
+      equal_nnz_partition_host(nparts, n, p->nnz, sp.data(), counts.data(),
+                               displs.data());
+      std::vector<int> wp;
+      wp.push_back(0);
+      for (int q = 0; q < nparts; q++) {
+        if (counts[q] <= 0) continue;
+        // split parts larger than the LDS window
+        int beg = displs[q], len = counts[q];
+        while (len > 0) {
+          const int take = std::min(len, c.window);
+          beg += take;
+          len -= take;
+          wp.push_back(beg);
+        }
+      }
+      while (wp.back() < n) wp.push_back(std::min(n, wp.back() + c.window));
+      ws->nwin = (int)wp.size() - 1;
+      ws->maxw = 0;
+      for (int q = 0; q < ws->nwin; q++)
+        ws->maxw = std::max(ws->maxw, wp[q + 1] - wp[q]);
+      st = ws->winPtr.ensure(sizeof(int) * wp.size(), false);
+      if (st == SPTRSV_OK &&
+          hipMemcpy(ws->winPtr.p, wp.data(), sizeof(int) * wp.size(),
+                    hipMemcpyHostToDevice) != hipSuccess)
+        st = SPTRSV_ERR_HIP;
+    }
+  }
+  if (st == SPTRSV_OK)
+    st = ws->need.ensure(sizeof(int2) * (size_t)std::max(n, 1), false);
+  if (st == SPTRSV_OK && n > 0) {
+    if (hipMemsetAsync(ws->need.p, 0, sizeof(int2) * (size_t)n, s) != hipSuccess)
+      st = SPTRSV_ERR_HIP;
+    else {
+      k_need<<<grid_for(n, 256), 256, 0, s>>>(
+          p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+          ws->winPtr.as<int>(), ws->nwin,
+          c.partition == SPTRSV_COL_PARTITION ? c.window : 0,
+          ws->need.as<int2>());
+      if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
+    }
+  }
+  if (st != SPTRSV_OK) {
+    ws->winPtr.release();
+    ws->need.release();
+    delete ws;
+    return st;
+  }
+  p->windows[key] = ws;
+  *out = ws;
+  return SPTRSV_OK;
+}
+
+// Everything a solve launch needs, prepared outside any graph capture.
+struct Launch {
+  sptrsv_config_t cfg;
+  WindowSet *ws = nullptr;
+  int ntiles = 0, R = 0, grid = 0;
+  size_t lds = 0;
+};
+
+template <typename T>
+int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
+                 hipStream_t s) {
+  const long long key = (long long)rhs * 2 + (f32solve ? 1 : 0);
+  auto it = p->configs.find(key);
+  L->cfg = it != p->configs.end() ? it->second : default_config(p, rhs);
+  const sptrsv_config_t &c = L->cfg;
+  ST_TRY(check_config(p, c, sizeof(T)));
+  ST_TRY(get_windows(p, c, &L->ws, s));
+  L->R = c.rhs_lanes * c.rhs_per_lane;
+  L->ntiles = (rhs + L->R - 1) / L->R;
+  L->lds = (size_t)L->ws->maxw * ((size_t)L->R * sizeof(T) + sizeof(int));
+  const size_t gsum_bytes =
+      std::max<size_t>(1, (size_t)L->ntiles * p->n * L->R) * sizeof(T);
+  const size_t gcnt_bytes =
+      std::max<size_t>(1, (size_t)L->ntiles * p->n) * sizeof(int);
+  ST_TRY(p->gsum.ensure(gsum_bytes, true));
+  ST_TRY(p->gcnt.ensure(gcnt_bytes, true));
+  const int maxrpl = c.rhs_per_lane == 1 ? 1 : 4;
+  ExecKernel<T> kern = pick_exec_dir<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+                                        c.lanes_per_column, c.rhs_lanes, maxrpl);
+  if (!kern) return SPTRSV_ERR_INVALID_ARG;
+  if (L->lds > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)kern,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)L->lds));
+  const int total = L->ws->nwin * L->ntiles;
+  int grid = c.grid;
+  if (grid <= 0) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, c.block,
+                                                         L->lds));
+    grid = std::max(1, per_cu) * p->cus;
+  }
+  L->grid = std::max(1, std::min(grid, std::max(total, 1)));
+  return SPTRSV_OK;
+}
+
+template <typename T>
+int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
+                T *x, int rhs, hipStream_t s) {
+  if (p->n == 0) return SPTRSV_OK;
+  const sptrsv_config_t &c = L.cfg;
+  ExecArgs<T> a;
+  a.colPtr = p->colPtr;
+  a.rowIdx = p->rowIdx;
+  a.val = val;
+  a.need = L.ws->need.as<int2>();
+  a.winPtr = L.ws->winPtr.as<int>();
+  a.gsum = p->gsum.as<T>();
+  a.gcnt = p->gcnt.as<int>();
+  a.b = b;
+  a.x = x;
+  a.n = p->n;
+  a.rhs = rhs;
+  a.nwin = L.ws->nwin;
+  a.ntiles = L.ntiles;
+  a.maxw = L.ws->maxw;
+  a.rpl = c.rhs_per_lane;
+  a.ticket = p->sync.as<int>();
+  a.err = p->sync.as<int>() + 2;
+  a.spin_cap = p->spin_cap;
+  const int maxrpl = c.rhs_per_lane == 1 ? 1 : 4;
+  ExecKernel<T> kern = pick_exec_dir<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+                                        c.lanes_per_column, c.rhs_lanes, maxrpl);
+  kern<<<L.grid, c.block, L.lds, s>>>(a);
+  HIP_TRY(hipGetLastError());
+  return SPTRSV_OK;
+}
+
+int reset_state(sptrsv_plan_s *p, hipStream_t s) {
+  if (p->gsum.p) HIP_TRY(hipMemsetAsync(p->gsum.p, 0, p->gsum.bytes, s));
+  if (p->gcnt.p) HIP_TRY(hipMemsetAsync(p->gcnt.p, 0, p->gcnt.bytes, s));
+  HIP_TRY(hipMemsetAsync(p->sync.p, 0, p->sync.bytes, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SPTRSV_OK;
+}
+
+int ensure_mixed_buffers(sptrsv_plan_s *p, int rhs) {
+  const size_t len = std::max<size_t>(1, (size_t)p->n * rhs);
+  ST_TRY(p->xtmp.ensure(len * sizeof(double), false));
+  ST_TRY(p->r32.ensure(len * sizeof(float), false));
+  ST_TRY(p->d32.ensure(len * sizeof(float), false));
+  return SPTRSV_OK;
+}
+
+// FP32 solve + FP64 refinement; all kernels on one stream, capturable.
+int launch_ir(sptrsv_plan_s *p, const Launch &L, const double *b, double *x,
+              int rhs, int iters, hipStream_t s) {
+  const size_t len = (size_t)p->n * rhs;
+  if (len == 0) return SPTRSV_OK;
+  float *r32 = p->r32.as<float>();
+  float *d32 = p->d32.as<float>();
+  const float *v32 = p->val32.as<float>();
+  k_to_f32<<<grid_for(len, 256), 256, 0, s>>>(b, r32, len);
+  HIP_TRY(hipGetLastError());
+  ST_TRY(launch_exec<float>(p, L, v32, r32, d32, rhs, s));
+  // ping-pong between the caller's x and the plan's scratch so that the last
+  // update lands in x
+  double *bufs[2] = {x, p->xtmp.as<double>()};
+  int cur = 0;
+  const double *x_old = nullptr;
+  for (int it = 1; it <= iters; it++) {
+    double *x_new = bufs[cur];
+    k_residual_update<<<grid_for(len, 256), 256, 0, s>>>(
+        p->csrPtr.as<int>(), p->csrIdx.as<int>(), p->csrVal.as<double>(),
+        p->n, rhs, b, x_old, d32, x_new, r32);
+    HIP_TRY(hipGetLastError());
+    ST_TRY(launch_exec<float>(p, L, v32, r32, d32, rhs, s));
+    x_old = x_new;
+    cur ^= 1;
+  }
+  k_update<<<grid_for(len, 256), 256, 0, s>>>(x_old, d32, x, len);
+  HIP_TRY(hipGetLastError());
+  return SPTRSV_OK;
+}
+
+int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
+                     const int *rowIdx, const void *val, int vt, int subst,
+                     int flags, hipStream_t s) {
+  if (!out || n < 0 || nnz < 0) return SPTRSV_ERR_INVALID_ARG;
+  if (n > 0 && (!colPtr || !rowIdx || !val)) return SPTRSV_ERR_INVALID_ARG;
+  if (vt != SPTRSV_F64 && vt != SPTRSV_F32) return SPTRSV_ERR_INVALID_ARG;
+  if (subst != SPTRSV_SUBSTITUTION_FORWARD &&
+      subst != SPTRSV_SUBSTITUTION_BACKWARD)
+    return SPTRSV_ERR_INVALID_ARG;
+  if ((flags & SPTRSV_PLAN_MIXED) && vt != SPTRSV_F64)
+    return SPTRSV_ERR_UNSUPPORTED;
+  *out = nullptr;
+  // the executor trusts the structure (a malformed factor would spin or read
+  // out of bounds), so it is always checked, whatever the flags say
+  ST_TRY(validate_impl(colPtr, rowIdx, val, vt, n, nnz, subst, s));
+  sptrsv_plan_s *p = new sptrsv_plan_s();
+  p->n = n;
+  p->nnz = nnz;
+  p->subst = subst;
+  p->vt = vt;
+  p->flags = flags;
+  p->colPtr = colPtr;
+  p->rowIdx = rowIdx;
+  p->val = val;
+  p->avg_offdiag = n > 0 ? (double)(nnz - n) / n : 0.0;
+  int st = device_props(&p->cus, &p->lds_max);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == SPTRSV_OK && (hipEventCreate(&e0) != hipSuccess ||
+                          hipEventCreate(&e1) != hipSuccess))
+    st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK) st = p->sync.ensure(4 * sizeof(int), true);
+  if (st == SPTRSV_OK) st = p->inDegree.ensure(sizeof(int) * std::max(n, 1), false);
+  if (st == SPTRSV_OK && hipEventRecord(e0, s) != hipSuccess) st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK && n > 0) {
+    // the reference's analyser: row histogram of the CSC row indices
+    if (hipMemsetAsync(p->inDegree.p, 0, sizeof(int) * n, s) != hipSuccess)
+      st = SPTRSV_ERR_HIP;
+    else if (nnz > 0) {
+      k_histogram<<<grid_for(nnz, 256), 256, 0, s>>>(rowIdx, nnz,
+                                                     p->inDegree.as<int>());
+      if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
+    }
+  }
+  if (st == SPTRSV_OK && (flags & SPTRSV_PLAN_MIXED) && n > 0) {
+    st = p->val32.ensure(sizeof(float) * std::max(nnz, 1), false);
+    if (st == SPTRSV_OK) st = p->csrPtr.ensure(sizeof(int) * (n + 1), false);
+    if (st == SPTRSV_OK) st = p->csrIdx.ensure(sizeof(int) * std::max(nnz, 1), false);
+    if (st == SPTRSV_OK) st = p->csrVal.ensure(sizeof(double) * std::max(nnz, 1), false);
+    if (st == SPTRSV_OK && nnz > 0) {
+      k_to_f32<<<grid_for(nnz, 256), 256, 0, s>>>((const double *)val,
+                                                  p->val32.as<float>(), nnz);
+      if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
+    }
+    // CSC of L read as CSR of L^T; transposing it gives the CSR of L
+    if (st == SPTRSV_OK)
+      st = csr2csc_impl(n, n, nnz, colPtr, rowIdx, val, SPTRSV_F64,
+                        p->csrPtr.as<int>(), p->csrIdx.as<int>(),
+                        p->csrVal.p, s);
+  }
+  if (st == SPTRSV_OK && hipEventRecord(e1, s) != hipSuccess) st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK && hipEventSynchronize(e1) != hipSuccess) st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    p->analysis_ms = ms;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st != SPTRSV_OK) {
+    sptrsv_plan_destroy(p);
+    return st;
+  }
+  *out = p;
+  return SPTRSV_OK;
+}
+
+std::string tune_key(const sptrsv_plan_s *p, int rhs, int mode) {
+  char buf[160];
+  snprintf(buf, sizeof(buf), "n=%d nnz=%d rhs=%d vt=%d mode=%d subst=%d nsys=%d",
+           p->n, p->nnz, rhs, p->vt, mode, p->subst, p->nsys);
+  return buf;
+}
+
+bool tune_cache_lookup(const std::string &key, sptrsv_config_t *c) {
+  const char *path = getenv("SPTRSV_TUNE_CACHE");
+  if (!path || !*path) return false;
+  FILE *f = fopen(path, "r");
+  if (!f) return false;
+  char line[512];
+  bool found = false;
+  while (fgets(line, sizeof(line), f)) {
+    char *bar = strchr(line, '|');
+    if (!bar) continue;
+    *bar = 0;
+    if (key != line) continue;
+    sptrsv_config_t t;
+    memset(&t, 0, sizeof(t));
+    if (sscanf(bar + 1, "%d %d %d %d %d %d %d", &t.lanes_per_column,
+               &t.rhs_lanes, &t.rhs_per_lane, &t.block, &t.window,
+               &t.partition, &t.grid) == 7) {
+      *c = t;
+      found = true;  // last entry wins
+    }
+  }
+  fclose(f);
+  return found;
+}
+
+void tune_cache_store(const std::string &key, const sptrsv_config_t &c) {
+  const char *path = getenv("SPTRSV_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s|%d %d %d %d %d %d %d\n", key.c_str(), c.lanes_per_column,
+          c.rhs_lanes, c.rhs_per_lane, c.block, c.window, c.partition, c.grid);
+  fclose(f);
+}
+
+std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
+                                             int tsize) {
+  std::vector<sptrsv_config_t> out;
+  auto add = [&](int G, int RL, int rpl, int block, int window, int part) {
+    sptrsv_config_t c;
+    memset(&c, 0, sizeof(c));
+    c.lanes_per_column = G;
+    c.rhs_lanes = RL;
+    c.rhs_per_lane = rpl;
+    c.block = block;
+    c.window = window;
+    c.partition = part;
+    if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
+  };
+  if (rhs <= 1) {
+    for (int G : {1, 2, 4, 8, 16, 64})
+      for (int block : {256, 1024}) {
+        const int groups = block / G;
+        for (int mult : {1, 4, 16}) add(G, 1, 1, block, groups * mult, 0);
+      }
+  } else {
+    int rl_cap = 1;  // smallest power of two covering rhs, at most 64
+    while (rl_cap < rhs && rl_cap < 64) rl_cap <<= 1;
+    for (int G = 1; G <= 64; G <<= 1) {
+      for (int RL : {1, G}) {
+        if (RL == G && G == 1) continue;  // (1, 1) is the RL = 1 case
+        if (RL > rl_cap) continue;
+        for (int rpl : {1, 2, 4}) {
+          if (rpl > 1 && RL * rpl >= 2 * rhs) continue;
+          for (int mult : {1, 4}) add(G, RL, rpl, 256, (256 / G) * mult, 0);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+template <typename T>
+int time_config(sptrsv_plan_s *p, const sptrsv_config_t &c, int rhs,
+                bool f32solve, const T *val, const T *b, T *x, float *ms,
+                hipStream_t s) {
+  const long long key = (long long)rhs * 2 + (f32solve ? 1 : 0);
+  auto saved = p->configs.find(key);
+  bool had = saved != p->configs.end();
+  sptrsv_config_t old = had ? saved->second : sptrsv_config_t();
+  p->configs[key] = c;
+  Launch L;
+  int st = prepare_exec<T>(p, rhs, f32solve, &L, s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == SPTRSV_OK && (hipEventCreate(&e0) != hipSuccess ||
+                          hipEventCreate(&e1) != hipSuccess))
+    st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK) st = launch_exec<T>(p, L, val, b, x, rhs, s);  // warm-up
+  if (st == SPTRSV_OK) st = sptrsv_plan_check(p, s);
+  const int reps = 5;
+  if (st == SPTRSV_OK) {
+    hipEventRecord(e0, s);
+    for (int r = 0; r < reps && st == SPTRSV_OK; r++)
+      st = launch_exec<T>(p, L, val, b, x, rhs, s);
+    hipEventRecord(e1, s);
+    if (st == SPTRSV_OK) st = sptrsv_plan_check(p, s);
+    float t = 0.f;
+    if (st == SPTRSV_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+      *ms = t / reps;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (had)
+    p->configs[key] = old;
+  else
+    p->configs.erase(key);
+  return st;
+}
+
+}  // namespace
+
+// =========================================================== C ABI: misc ===
+extern "C" {
+
+const char *sptrsv_status_string(int st) {
+  switch (st) {
+    case SPTRSV_OK: return "ok";
+    case SPTRSV_ERR_NOT_SQUARE: return "matrix is not square";
+    case SPTRSV_ERR_INVALID_ARG: return "invalid argument";
+    case SPTRSV_ERR_MISSING_DIAG: return "missing diagonal entry";
+    case SPTRSV_ERR_ZERO_DIAG: return "zero diagonal entry";
+    case SPTRSV_ERR_NOT_TRIANGULAR: return "entry outside the triangle";
+    case SPTRSV_ERR_UNSORTED: return "column rows not ascending";
+    case SPTRSV_ERR_INDEX: return "index out of range";
+    case SPTRSV_ERR_HIP: return "HIP runtime error";
+    case SPTRSV_ERR_TIMEOUT: return "spin-wait timeout";
+    case SPTRSV_ERR_NO_DEVICE: return "no HIP device";
+    case SPTRSV_ERR_UNSUPPORTED: return "unsupported";
+    case SPTRSV_ERR_NO_MEMORY: return "out of device memory";
+    default: return "unknown status";
+  }
+}
+
+const char *sptrsv_version(void) { return "0.1.0 gfx950"; }
+
+void sptrsv_set_verbose(int v) { g_verbose = v; }
+void sptrsv_set_bench_repeat(int r) { g_bench_repeat = r < 1 ? 1 : r; }
+
+int sptrsv_device_count(int *count) {
+  if (!count) return SPTRSV_ERR_INVALID_ARG;
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e == hipErrorNoDevice || (e == hipSuccess && *count == 0)) {
+    *count = 0;
+    return SPTRSV_ERR_NO_DEVICE;
+  }
+  HIP_TRY(e);
+  return SPTRSV_OK;
+}
+int sptrsv_set_device(int d) { HIP_TRY(hipSetDevice(d)); return SPTRSV_OK; }
+int sptrsv_device_name(char *buf, int len) {
+  if (!buf || len <= 0) return SPTRSV_ERR_INVALID_ARG;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  return SPTRSV_OK;
+}
+int sptrsv_device_cu_count(int *cus) {
+  if (!cus) return SPTRSV_ERR_INVALID_ARG;
+  size_t l = 0;
+  return device_props(cus, &l);
+}
+int sptrsv_malloc(void **d, size_t bytes) {
+  if (!d) return SPTRSV_ERR_INVALID_ARG;
+  HIP_TRY(hipMalloc(d, bytes == 0 ? 1 : bytes));
+  return SPTRSV_OK;
+}
+int sptrsv_free(void *d) { if (d) HIP_TRY(hipFree(d)); return SPTRSV_OK; }
+int sptrsv_memcpy_htod(void *dst, const void *src, size_t b) {
+  if (b) HIP_TRY(hipMemcpy(dst, src, b, hipMemcpyHostToDevice));
+  return SPTRSV_OK;
+}
+int sptrsv_memcpy_dtoh(void *dst, const void *src, size_t b) {
+  if (b) HIP_TRY(hipMemcpy(dst, src, b, hipMemcpyDeviceToHost));
+  return SPTRSV_OK;
+}
+int sptrsv_memcpy_dtod(void *dst, const void *src, size_t b) {
+  if (b) HIP_TRY(hipMemcpy(dst, src, b, hipMemcpyDeviceToDevice));
+  return SPTRSV_OK;
+}
+int sptrsv_memset(void *d, int v, size_t b) {
+  if (b) HIP_TRY(hipMemset(d, v, b));
+  return SPTRSV_OK;
+}
+int sptrsv_synchronize(void) { HIP_TRY(hipDeviceSynchronize()); return SPTRSV_OK; }
+int sptrsv_stream_create(void **s) {
+  if (!s) return SPTRSV_ERR_INVALID_ARG;
+  hipStream_t h;
+  HIP_TRY(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+  *s = h;
+  return SPTRSV_OK;
+}
+int sptrsv_stream_destroy(void *s) { if (s) HIP_TRY(hipStreamDestroy(as_stream(s))); return SPTRSV_OK; }
+int sptrsv_stream_synchronize(void *s) { HIP_TRY(hipStreamSynchronize(as_stream(s))); return SPTRSV_OK; }
+int sptrsv_event_create(void **e) {
+  if (!e) return SPTRSV_ERR_INVALID_ARG;
+  hipEvent_t h;
+  HIP_TRY(hipEventCreate(&h));
+  *e = h;
+  return SPTRSV_OK;
+}
+int sptrsv_event_destroy(void *e) { if (e) HIP_TRY(hipEventDestroy((hipEvent_t)e)); return SPTRSV_OK; }
+int sptrsv_event_record(void *e, void *s) { HIP_TRY(hipEventRecord((hipEvent_t)e, as_stream(s))); return SPTRSV_OK; }
+int sptrsv_event_elapsed(void *a, void *b, float *ms) {
+  if (!ms) return SPTRSV_ERR_INVALID_ARG;
+  HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+  HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return SPTRSV_OK;
+}
+
+// ===================================================== C ABI: building blocks
+int sptrsv_exclusive_scan(int *d, int len, void *stream) {
+  if (len < 0 || (len > 0 && !d)) return SPTRSV_ERR_INVALID_ARG;
+  return scan_rec(d, len, as_stream(stream));
+}
+
+int sptrsv_csr2csc(int m, int n, int nnz, const int *rowPtr, const int *colIdx,
+                   const void *val, int vt, int *colPtr, int *rowIdx,
+                   void *cscVal, void *stream) {
+  if (m < 0 || n < 0 || nnz < 0) return SPTRSV_ERR_INVALID_ARG;
+  if (!colPtr || (nnz > 0 && (!rowPtr || !colIdx || !rowIdx)))
+    return SPTRSV_ERR_INVALID_ARG;
+  if ((val == nullptr) != (cscVal == nullptr)) return SPTRSV_ERR_INVALID_ARG;
+  if (vt != SPTRSV_F64 && vt != SPTRSV_F32) return SPTRSV_ERR_INVALID_ARG;
+  return csr2csc_impl(m, n, nnz, rowPtr, colIdx, val, vt, colPtr, rowIdx,
+                      cscVal, as_stream(stream));
+}
+
+int sptrsv_analyse(const int *rowIdx, int m, int nnz, int *inDegree,
+                   void *stream) {
+  if (m < 0 || nnz < 0 || (m > 0 && !inDegree) || (nnz > 0 && !rowIdx))
+    return SPTRSV_ERR_INVALID_ARG;
+  hipStream_t s = as_stream(stream);
+  if (m > 0) HIP_TRY(hipMemsetAsync(inDegree, 0, sizeof(int) * m, s));
+  if (nnz > 0) {
+    k_histogram<<<grid_for(nnz, 256), 256, 0, s>>>(rowIdx, nnz, inDegree);
+    HIP_TRY(hipGetLastError());
+  }
+  return SPTRSV_OK;
+}
+
+int sptrsv_validate(const int *colPtr, const int *rowIdx, const void *val,
+                    int vt, int n, int nnz, int subst, void *stream) {
+  if (n < 0 || nnz < 0 || (n > 0 && (!colPtr || !rowIdx)))
+    return SPTRSV_ERR_INVALID_ARG;
+  if (subst != SPTRSV_SUBSTITUTION_FORWARD &&
+      subst != SPTRSV_SUBSTITUTION_BACKWARD)
+    return SPTRSV_ERR_INVALID_ARG;
+  return validate_impl(colPtr, rowIdx, val, vt, n, nnz, subst, as_stream(stream));
+}
+
+int sptrsv_equal_col_partition(int nparts, int cols, int *counts, int *displs) {
+  if (nparts < 1 || cols < 0 || !counts || !displs) return SPTRSV_ERR_INVALID_ARG;
+  const int each = (int)round((double)cols / nparts);
+  for (int q = 0; q + 1 < nparts; q++) {
+    counts[q] = each;
+    displs[q] = q * each;
+  }
+  counts[nparts - 1] = cols - (nparts - 1) * each;
+  displs[nparts - 1] = (nparts - 1) * each;
+  return SPTRSV_OK;
+}
+
+int sptrsv_equal_nnz_partition(int nparts, int ncols, int nnz,
+                               const int *colPtr, int *counts, int *displs) {
+  if (nparts < 1 || ncols < 0 || nnz < 0 || !colPtr || !counts || !displs)
+    return SPTRSV_ERR_INVALID_ARG;
+  for (int q = 0; q < nparts; q++) counts[q] = displs[q] = 0;
+  equal_nnz_partition_host(nparts, ncols, nnz, colPtr, counts, displs);
+  return SPTRSV_OK;
+}
+
+int sptrsv_levels(const int *colPtr, const int *rowIdx, int n, int nnz,
+                  int *nlevel, int *pmin, int *pavg, int *pmax) {
+  if (n < 0 || !colPtr || (nnz > 0 && !rowIdx) || !nlevel || !pmin || !pavg ||
+      !pmax)
+    return SPTRSV_ERR_INVALID_ARG;
+  if (n == 0) {
+    *nlevel = *pmin = *pavg = *pmax = 0;
+    return SPTRSV_OK;
+  }
+  // Kahn BFS from rows whose only entry is the diagonal (findlevel.h:6-69)
+  std::vector<int> indeg(n, 0), items(n), lptr;
+  for (int j = 0; j < nnz; j++) {
+    if (rowIdx[j] < 0 || rowIdx[j] >= n) return SPTRSV_ERR_INDEX;
+    indeg[rowIdx[j]]++;
+  }
+  int filled = 0;
+  lptr.push_back(0);
+  for (int i = 0; i < n; i++)
+    if (indeg[i] == 1) items[filled++] = i;
+  lptr.push_back(filled);
+  while (lptr.back() != n) {
+    const int a = lptr[lptr.size() - 2], b = lptr.back();
+    for (int t = a; t < b; t++) {
+      const int node = items[t];
+      for (int j = colPtr[node]; j < colPtr[node + 1]; j++) {
+        const int v = rowIdx[j];
+        if (--indeg[v] == 1) items[filled++] = v;
+      }
+    }
+    if (filled == lptr.back()) return SPTRSV_ERR_INVALID_ARG;  // not a DAG
+    lptr.push_back(filled);
+  }
+  const int nl = (int)lptr.size() - 1;
+  int mn = lptr[1] - lptr[0], mx = mn;
+  for (int l = 1; l < nl; l++) {
+    const int w = lptr[l + 1] - lptr[l];
+    if (w > mx)
+      mx = w;
+    else if (w < mn)  // findlevel.h keeps this else-if; so do we
+      mn = w;
+  }
+  *nlevel = nl;
+  *pmin = mn;
+  *pmax = mx;
+  *pavg = n / nl;
+  return SPTRSV_OK;
+}
+
+// ============================================================ C ABI: plans ==
+int sptrsv_plan_create(sptrsv_plan_t *plan, int n, int nnz, const int *colPtr,
+                       const int *rowIdx, const void *val, int vt, int subst,
+                       int flags, void *stream) {
+  return plan_create_impl(plan, n, nnz, colPtr, rowIdx, val, vt, subst, flags,
+                          as_stream(stream));
+}
+
+int sptrsv_batch_plan_create(sptrsv_plan_t *plan, int nsys, const int *sys_ptr,
+                             int n, int nnz, const int *colPtr,
+                             const int *rowIdx, const void *val, int vt,
+                             int subst, int flags, void *stream) {
+  if (!plan || nsys < 1 || !sys_ptr || sys_ptr[0] != 0 || sys_ptr[nsys] != n)
+    return SPTRSV_ERR_INVALID_ARG;
+  for (int q = 0; q < nsys; q++)
+    if (sys_ptr[q + 1] < sys_ptr[q]) return SPTRSV_ERR_INVALID_ARG;
+  hipStream_t s = as_stream(stream);
+  // every entry of a column must stay inside the column's system
+  if (n > 0 && nnz > 0) {
+    std::vector<int> cp(n + 1), ri(nnz);
+    HIP_TRY(hipMemcpy(cp.data(), colPtr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ri.data(), rowIdx, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    if (cp[0] != 0 || cp[n] != nnz) return SPTRSV_ERR_INDEX;
+    for (int q = 0; q < nsys; q++)
+      for (int c = sys_ptr[q]; c < sys_ptr[q + 1]; c++)
+        for (int j = cp[c]; j < cp[c + 1]; j++)
+          if (ri[j] < sys_ptr[q] || ri[j] >= sys_ptr[q + 1])
+            return SPTRSV_ERR_INVALID_ARG;
+  }
+  ST_TRY(plan_create_impl(plan, n, nnz, colPtr, rowIdx, val, vt, subst, flags, s));
+  (*plan)->nsys = nsys;
+  (*plan)->sys_ptr.assign(sys_ptr, sys_ptr + nsys + 1);
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_destroy(sptrsv_plan_t p) {
+  if (!p) return SPTRSV_OK;
+  for (auto &kv : p->windows) {
+    kv.second->winPtr.release();
+    kv.second->need.release();
+    delete kv.second;
+  }
+  p->windows.clear();
+  DevBuf *bufs[] = {&p->inDegree, &p->sync, &p->gsum,   &p->gcnt, &p->val32,
+                    &p->csrPtr,   &p->csrIdx, &p->csrVal, &p->xtmp, &p->r32,
+                    &p->d32};
+  for (DevBuf *b : bufs) b->release();
+  delete p;
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg) {
+  if (!p || !cfg || rhs < 1) return SPTRSV_ERR_INVALID_ARG;
+  sptrsv_config_t c = *cfg;
+  const sptrsv_config_t d = default_config(p, rhs);
+  if (c.lanes_per_column == 0) c.lanes_per_column = d.lanes_per_column;
+  if (c.rhs_lanes == 0) c.rhs_lanes = std::min(d.rhs_lanes, c.lanes_per_column);
+  if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
+  if (c.block == 0) c.block = d.block;
+  if (c.window == 0) c.window = c.block / c.lanes_per_column;
+  // the same configuration serves the plan's own type and the FP32 solve of IR
+  const int tsize = p->vt == SPTRSV_F64 ? 8 : 4;
+  ST_TRY(check_config(p, c, tsize));
+  p->configs[(long long)rhs * 2 + (p->vt == SPTRSV_F32 ? 1 : 0)] = c;
+  if (p->vt == SPTRSV_F64) p->configs[(long long)rhs * 2 + 1] = c;
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_get_config(sptrsv_plan_t p, int rhs, sptrsv_config_t *cfg) {
+  if (!p || !cfg || rhs < 1) return SPTRSV_ERR_INVALID_ARG;
+  auto it = p->configs.find((long long)rhs * 2 + (p->vt == SPTRSV_F32 ? 1 : 0));
+  *cfg = it != p->configs.end() ? it->second : default_config(p, rhs);
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_solve(sptrsv_plan_t p, const void *b, void *x, int rhs,
+                      void *stream) {
+  if (!p || rhs < 1 || (p->n > 0 && (!b || !x))) return SPTRSV_ERR_INVALID_ARG;
+  hipStream_t s = as_stream(stream);
+  Launch L;
+  if (p->vt == SPTRSV_F64) {
+    ST_TRY(prepare_exec<double>(p, rhs, false, &L, s));
+    return launch_exec<double>(p, L, (const double *)p->val, (const double *)b,
+                               (double *)x, rhs, s);
+  }
+  ST_TRY(prepare_exec<float>(p, rhs, true, &L, s));
+  return launch_exec<float>(p, L, (const float *)p->val, (const float *)b,
+                            (float *)x, rhs, s);
+}
+
+int sptrsv_plan_solve_ir(sptrsv_plan_t p, const double *b, double *x, int rhs,
+                         int iters, void *stream) {
+  if (!p || rhs < 1 || iters < 0 || (p->n > 0 && (!b || !x)))
+    return SPTRSV_ERR_INVALID_ARG;
+  if (!(p->flags & SPTRSV_PLAN_MIXED) || p->vt != SPTRSV_F64)
+    return SPTRSV_ERR_UNSUPPORTED;
+  hipStream_t s = as_stream(stream);
+  Launch L;
+  ST_TRY(prepare_exec<float>(p, rhs, true, &L, s));
+  ST_TRY(ensure_mixed_buffers(p, rhs));
+  return launch_ir(p, L, b, x, rhs, iters, s);
+}
+
+int sptrsv_plan_check(sptrsv_plan_t p, void *stream) {
+  if (!p) return SPTRSV_ERR_INVALID_ARG;
+  hipStream_t s = as_stream(stream);
+  int err = 0;
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipMemcpy(&err, p->sync.as<int>() + 2, sizeof(int), hipMemcpyDeviceToHost));
+  if (err != 0) {
+    ST_TRY(reset_state(p, s));
+    return SPTRSV_ERR_TIMEOUT;
+  }
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_reset(sptrsv_plan_t p, void *stream) {
+  if (!p) return SPTRSV_ERR_INVALID_ARG;
+  return reset_state(p, as_stream(stream));
+}
+
+int sptrsv_plan_set_spin_cap(sptrsv_plan_t p, long long cap) {
+  if (!p || cap < 0) return SPTRSV_ERR_INVALID_ARG;
+  p->spin_cap = cap == 0 ? kDefaultSpinCap : cap;
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
+  if (!p || !info) return SPTRSV_ERR_INVALID_ARG;
+  memset(info, 0, sizeof(*info));
+  info->n = p->n;
+  info->nnz = p->nnz;
+  info->substitution = p->subst;
+  info->value_type = p->vt;
+  info->nsys = p->nsys;
+  long long bytes = 0;
+  const DevBuf *bufs[] = {&p->inDegree, &p->sync, &p->gsum, &p->gcnt,
+                          &p->val32,    &p->csrPtr, &p->csrIdx, &p->csrVal,
+                          &p->xtmp,     &p->r32,  &p->d32};
+  for (const DevBuf *b : bufs) bytes += (long long)b->bytes;
+  for (auto &kv : p->windows)
+    bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes);
+  info->scratch_bytes = bytes;
+  info->analysis_ms = p->analysis_ms;
+  return SPTRSV_OK;
+}
+
+int sptrsv_plan_autotune(sptrsv_plan_t p, int rhs, int mode,
+                         sptrsv_config_t *best, float *best_ms) {
+  if (!p || rhs < 1 || (mode != 0 && mode != 1)) return SPTRSV_ERR_INVALID_ARG;
+  if (mode == 1 && (!(p->flags & SPTRSV_PLAN_MIXED) || p->vt != SPTRSV_F64))
+    return SPTRSV_ERR_UNSUPPORTED;
+  const bool f32 = mode == 1 || p->vt == SPTRSV_F32;
+  const long long key = (long long)rhs * 2 + (f32 ? 1 : 0);
+  const std::string ck = tune_key(p, rhs, mode);
+  sptrsv_config_t cached;
+  if (tune_cache_lookup(ck, &cached) &&
+      check_config(p, cached, f32 ? 4 : 8) == SPTRSV_OK) {
+    p->configs[key] = cached;
+    if (best) *best = cached;
+    if (best_ms) *best_ms = -1.f;
+    return SPTRSV_OK;
+  }
+  if (p->n == 0) return SPTRSV_OK;
+  hipStream_t s = nullptr;
+  const size_t len = (size_t)p->n * rhs;
+  DevBuf bb, xx;
+  const size_t tsize = f32 ? 4 : 8;
+  ST_TRY(bb.ensure(len * tsize, true));
+  int st = xx.ensure(len * tsize, true);
+  if (st != SPTRSV_OK) {
+    bb.release();
+    return st;
+  }
+  const void *val = p->val;
+  if (mode == 1) val = p->val32.p;
+  // b = 1 everywhere: any value works, timing does not depend on it
+  if (f32)
+    fill_async<float>(bb.as<float>(), len, 1.0f, s);
+  else
+    fill_async<double>(bb.as<double>(), len, 1.0, s);
+  std::vector<sptrsv_config_t> cands = tune_candidates(p, rhs, (int)tsize);
+  cands.insert(cands.begin(), default_config(p, rhs));
+  float bms = 1e30f;
+  sptrsv_config_t bc = cands[0];
+  for (const sptrsv_config_t &c : cands) {
+    float ms = 0.f;
+    int r = f32 ? time_config<float>(p, c, rhs, true, (const float *)val,
+                                     bb.as<float>(), xx.as<float>(), &ms, s)
+                : time_config<double>(p, c, rhs, false, (const double *)val,
+                                      bb.as<double>(), xx.as<double>(), &ms, s);
+    if (r == SPTRSV_OK && ms > 0.f && ms < bms) {
+      bms = ms;
+      bc = c;
+    }
+    if (g_verbose)
+      fprintf(stderr, "tune G=%d RL=%d rpl=%d block=%d window=%d : %s %.4f ms\n",
+              c.lanes_per_column, c.rhs_lanes, c.rhs_per_lane, c.block,
+              c.window, sptrsv_status_string(r), ms);
+  }
+  bb.release();
+  xx.release();
+  p->configs[key] = bc;
+  tune_cache_store(ck, bc);
+  if (best) *best = bc;
+  if (best_ms) *best_ms = bms;
+  return SPTRSV_OK;
+}
+
+// ========================================================== C ABI: graphs ==
+struct sptrsv_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int sptrsv_graph_create(sptrsv_graph_t *out, sptrsv_plan_t p, const void *b,
+                        void *x, int rhs, int mode, int iters, void *stream) {
+  if (!out || !p || rhs < 1 || (mode != 0 && mode != 1) || iters < 0)
+    return SPTRSV_ERR_INVALID_ARG;
+  hipStream_t user = as_stream(stream);
+  // prepare everything that allocates before capturing
+  Launch L;
+  if (mode == 1) {
+    if (!(p->flags & SPTRSV_PLAN_MIXED) || p->vt != SPTRSV_F64)
+      return SPTRSV_ERR_UNSUPPORTED;
+    ST_TRY(prepare_exec<float>(p, rhs, true, &L, user));
+    ST_TRY(ensure_mixed_buffers(p, rhs));
+  } else if (p->vt == SPTRSV_F64) {
+    ST_TRY(prepare_exec<double>(p, rhs, false, &L, user));
+  } else {
+    ST_TRY(prepare_exec<float>(p, rhs, true, &L, user));
+  }
+  HIP_TRY(hipStreamSynchronize(user));
+  hipStream_t cs;
+  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  sptrsv_graph_s *g = new sptrsv_graph_s();
+  int st = SPTRSV_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    st = SPTRSV_ERR_HIP;
+  if (st == SPTRSV_OK) {
+    int r;
+    if (mode == 1)
+      r = launch_ir(p, L, (const double *)b, (double *)x, rhs, iters, cs);
+    else if (p->vt == SPTRSV_F64)
+      r = launch_exec<double>(p, L, (const double *)p->val, (const double *)b,
+                              (double *)x, rhs, cs);
+    else
+      r = launch_exec<float>(p, L, (const float *)p->val, (const float *)b,
+                             (float *)x, rhs, cs);
+    hipError_t e = hipStreamEndCapture(cs, &g->graph);
+    if (r != SPTRSV_OK)
+      st = r;
+    else if (e != hipSuccess)
+      st = SPTRSV_ERR_HIP;
+  }
+  if (st == SPTRSV_OK &&
+      hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    st = SPTRSV_ERR_HIP;
+  hipStreamDestroy(cs);
+  if (st != SPTRSV_OK) {
+    sptrsv_graph_destroy(g);
+    return st;
+  }
+  *out = g;
+  return SPTRSV_OK;
+}
+
+int sptrsv_graph_launch(sptrsv_graph_t g, void *stream) {
+  if (!g || !g->exec) return SPTRSV_ERR_INVALID_ARG;
+  HIP_TRY(hipGraphLaunch(g->exec, as_stream(stream)));
+  return SPTRSV_OK;
+}
+
+int sptrsv_graph_destroy(sptrsv_graph_t g) {
+  if (!g) return SPTRSV_OK;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+  return SPTRSV_OK;
+}
+
+// ======================================================= C ABI: drop-ins ===
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+double rel_err_l1(const T *x, const T *xr, size_t len) {
+  double ref = 0.0, res = 0.0;
+  for (size_t i = 0; i < len; i++) {
+    ref += fabs((double)xr[i]);
+    res += fabs((double)x[i] - (double)xr[i]);
+  }
+  return ref == 0.0 ? res : res / ref;
+}
+
+// Shared body of the drop-in entry points: upload, analyse, time
+// g_bench_repeat solves, download, validate (sptrsv_syncfree_cuda.h:429-597).
+template <typename T>
+int dropin(const int *colPtr, const int *rowIdx, const T *val, int m, int n,
+           int nnz, int subst, int rhs, int opt, T *x, const T *b,
+           const T *x_ref, double *gflops, bool csr_in) {
+  if (m != n) {
+    if (g_verbose) printf("This is not a square matrix, return.\n");
+    return SPTRSV_ERR_NOT_SQUARE;
+  }
+  if (n < 0 || nnz < 0 || rhs < 1 || !x || !b || !colPtr ||
+      (nnz > 0 && (!rowIdx || !val)))
+    return SPTRSV_ERR_INVALID_ARG;
+  if (opt != SPTRSV_OPT_WARP_NNZ && opt != SPTRSV_OPT_WARP_RHS &&
+      opt != SPTRSV_OPT_WARP_AUTO)
+    return SPTRSV_ERR_INVALID_ARG;
+  if (subst != SPTRSV_SUBSTITUTION_FORWARD &&
+      subst != SPTRSV_SUBSTITUTION_BACKWARD)
+    return SPTRSV_ERR_INVALID_ARG;
+  int ndev = 0;
+  ST_TRY(sptrsv_device_count(&ndev));
+  const int vt = sizeof(T) == 8 ? SPTRSV_F64 : SPTRSV_F32;
+  const size_t len = (size_t)n * rhs;
+  DevBuf dPtr, dIdx, dVal, dB, dX, iPtr, iIdx, iVal;
+  sptrsv_plan_t plan = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = SPTRSV_OK;
+  do {
+    if ((st = dPtr.ensure(sizeof(int) * (n + 1), false)) ||
+        (st = dIdx.ensure(sizeof(int) * std::max(nnz, 1), false)) ||
+        (st = dVal.ensure(sizeof(T) * std::max(nnz, 1), false)) ||
+        (st = dB.ensure(sizeof(T) * std::max<size_t>(len, 1), false)) ||
+        (st = dX.ensure(sizeof(T) * std::max<size_t>(len, 1), true)))
+      break;
+    if (csr_in) {
+      // CSR in: upload and transpose on the device (main.cu:208-210)
+      if ((st = iPtr.ensure(sizeof(int) * (n + 1), false)) ||
+          (st = iIdx.ensure(sizeof(int) * std::max(nnz, 1), false)) ||
+          (st = iVal.ensure(sizeof(T) * std::max(nnz, 1), false)))
+        break;
+      if (hipMemcpy(iPtr.p, colPtr, sizeof(int) * (n + 1), hipMemcpyHostToDevice) ||
+          (nnz && hipMemcpy(iIdx.p, rowIdx, sizeof(int) * nnz, hipMemcpyHostToDevice)) ||
+          (nnz && hipMemcpy(iVal.p, val, sizeof(T) * nnz, hipMemcpyHostToDevice))) {
+        st = SPTRSV_ERR_HIP;
+        break;
+      }
+      int first_last[2] = {0, 0};
+      first_last[0] = colPtr[0];
+      first_last[1] = colPtr[n];
+      if (first_last[0] != 0 || first_last[1] != nnz) {
+        st = SPTRSV_ERR_INDEX;
+        break;
+      }
+      st = csr2csc_impl(n, n, nnz, iPtr.as<int>(), iIdx.as<int>(), iVal.p, vt,
+                        dPtr.as<int>(), dIdx.as<int>(), dVal.p, nullptr);
+      if (st) break;
+    } else {
+      if (hipMemcpy(dPtr.p, colPtr, sizeof(int) * (n + 1), hipMemcpyHostToDevice) ||
+          (nnz && hipMemcpy(dIdx.p, rowIdx, sizeof(int) * nnz, hipMemcpyHostToDevice)) ||
+          (nnz && hipMemcpy(dVal.p, val, sizeof(T) * nnz, hipMemcpyHostToDevice))) {
+        st = SPTRSV_ERR_HIP;
+        break;
+      }
+    }
+    if (len && hipMemcpy(dB.p, b, sizeof(T) * len, hipMemcpyHostToDevice)) {
+      st = SPTRSV_ERR_HIP;
+      break;
+    }
+    st = plan_create_impl(&plan, n, nnz, dPtr.as<int>(), dIdx.as<int>(), dVal.p,
+                          vt, subst, SPTRSV_PLAN_VALIDATE, nullptr);
+    if (st) break;
+    if (g_verbose)
+      printf("hip syncfree SpTRSV analysis on L used %4.2f ms\n",
+             plan->analysis_ms);
+    if (rhs > 1 && opt != SPTRSV_OPT_WARP_AUTO) {
+      sptrsv_config_t c = default_config(plan, rhs);
+      if (opt == SPTRSV_OPT_WARP_NNZ) {
+        c.lanes_per_column = 64;
+        c.rhs_lanes = 1;
+        c.rhs_per_lane = rhs >= 4 ? 4 : 1;
+        c.block = 256;
+        c.window = 4;
+      }
+      if ((st = sptrsv_plan_set_config(plan, rhs, &c))) break;
+    }
+    if (hipEventCreate(&e0) || hipEventCreate(&e1)) {
+      st = SPTRSV_ERR_HIP;
+      break;
+    }
+    double total_ms = 0.0;
+    for (int r = 0; r < g_bench_repeat && st == SPTRSV_OK; r++) {
+      hipEventRecord(e0, nullptr);
+      st = sptrsv_plan_solve(plan, dB.p, dX.p, rhs, nullptr);
+      hipEventRecord(e1, nullptr);
+      if (st) break;
+      st = sptrsv_plan_check(plan, nullptr);
+      float ms = 0.f;
+      if (st == SPTRSV_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+        total_ms += ms;
+    }
+    if (st) break;
+    const double t = total_ms / g_bench_repeat;
+    const double flop = 2.0 * (double)rhs * (double)nnz;
+    if (gflops) *gflops = t > 0.0 ? flop / (1e6 * t) : 0.0;
+    if (g_verbose)
+      printf("hip syncfree SpTRSV solve used %4.2f ms, throughput is %4.2f gflops\n",
+             t, t > 0.0 ? flop / (1e6 * t) : 0.0);
+    if (len && hipMemcpy(x, dX.p, sizeof(T) * len, hipMemcpyDeviceToHost)) {
+      st = SPTRSV_ERR_HIP;
+      break;
+    }
+    if (x_ref && g_verbose) {
+      const double res = rel_err_l1(x, x_ref, len);
+      if (res < 1e-4)
+        printf("hip syncfree SpTRSV executor passed! |x-xref|/|xref| = %8.2e\n", res);
+      else
+        printf("hip syncfree SpTRSV executor _NOT_ passed! |x-xref|/|xref| = %8.2e\n", res);
+    }
+  } while (0);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (plan) sptrsv_plan_destroy(plan);
+  DevBuf *bufs[] = {&dPtr, &dIdx, &dVal, &dB, &dX, &iPtr, &iIdx, &iVal};
+  for (DevBuf *bf : bufs) bf->release();
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sptrsv_syncfree_hip(const int *cscColPtrTR, const int *cscRowIdxTR,
+                        const double *cscValTR, const int m, const int n,
+                        const int nnzTR, const int substitution, const int rhs,
+                        const int opt, double *x, const double *b,
+                        const double *x_ref, double *gflops) {
+  return dropin<double>(cscColPtrTR, cscRowIdxTR, cscValTR, m, n, nnzTR,
+                        substitution, rhs, opt, x, b, x_ref, gflops, false);
+}
+
+int sptrsv_syncfree_hip_f32(const int *cscColPtrTR, const int *cscRowIdxTR,
+                            const float *cscValTR, const int m, const int n,
+                            const int nnzTR, const int substitution,
+                            const int rhs, const int opt, float *x,
+                            const float *b, const float *x_ref,
+                            double *gflops) {
+  return dropin<float>(cscColPtrTR, cscRowIdxTR, cscValTR, m, n, nnzTR,
+                       substitution, rhs, opt, x, b, x_ref, gflops, false);
+}
+
+int sptrsv_syncfree_hip_csr(const int *csrRowPtr, const int *csrColIdx,
+                            const double *csrVal, const int m, const int n,
+                            const int nnz, const int substitution,
+                            const int rhs, const int opt, double *x,
+                            const double *b, const double *x_ref,
+                            double *gflops) {
+  return dropin<double>(csrRowPtr, csrColIdx, csrVal, m, n, nnz, substitution,
+                        rhs, opt, x, b, x_ref, gflops, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// sptrsv_device.hpp -- shared device helpers for the gfx950 SpTRSV kernels.
+//
+// Memory-model notes (CDNA3/CDNA4, 8 XCDs with one L2 each):
+//  * Cross-workgroup data goes through agent-scope atomics.  A relaxed
+//    agent-scope load compiles to a global load with sc1 (served coherently
+//    across XCDs); an agent-scope release fence writes back the issuing XCD's
+//    L2 (buffer_wbl2 sc1) and an acquire fence invalidates it (buffer_inv sc1).
+//  * Within a workgroup, LDS atomics and workgroup-scope fences suffice.
+//  * Lanes of one wavefront execute memory instructions together, so a fence
+//    executed by a wavefront orders every lane's earlier accesses; the
+//    executors rely on this when one lane publishes a counter after all lanes
+//    of its group have added their partial sums.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sptrsv {
+
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void add_agent(T *p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_wg(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void add_wg(T *p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void fence_acq_agent() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+__device__ __forceinline__ void fence_rel_agent() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+}
+__device__ __forceinline__ void fence_acq_wg() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+__device__ __forceinline__ void fence_rel_wg() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
+
+// Largest index i in [0, len) with a[i] <= key (a ascending, a[0] <= key).
+__device__ __forceinline__ int upper_bound_minus1(const int *a, int len,
+                                                  int key) {
+  int lo = 0, hi = len;  // invariant: a[lo] <= key, answer in [lo, hi)
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= key)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace sptrsv

@@ -1,0 +1,267 @@
+/*
+ * sptrsv_hip.h -- C ABI of the MI355X (gfx950) sync-free sparse triangular
+ * solver.
+ *
+ * Plain C: no HIP, torch or C++ types cross this boundary.  Device pointers are
+ * `void *`/typed pointers obtained from hipMalloc (or sptrsv_malloc), streams
+ * are hipStream_t passed as `void *` (NULL = the library's default stream).
+ * Every entry point returns an int status (SPTRSV_OK == 0, negative on error).
+ *
+ * Reference interfaces replaced (paths relative to harunsasmaz/Benchmark-SpTRSV):
+ *   sptrsv_syncfree_hip          <- sptrsv_syncfree_cuda, SpTRSV_cuda/sptrsv_syncfree_cuda.h:409-421
+ *   sptrsv_syncfree_hip_f32      <- the same entry built with -D VALUE_TYPE=float
+ *                                   (SpTRSV_cuda/Makefile_macos:5, common.h:9-10)
+ *   sptrsv_syncfree_hip_csr      <- main.cu:208-210 (host matrix_transposition) + the call above
+ *   sptrsv_csr2csc               <- matrix_transposition, SpTRSV_cuda/tranpose.h:6-43
+ *   sptrsv_exclusive_scan        <- exclusive_scan, SpTRSV_cuda/utils.h:374-391
+ *   sptrsv_analyse               <- sptrsv_syncfree_cuda_analyser, sptrsv_syncfree_cuda.h:8-19
+ *   sptrsv_plan_* (solve)        <- the analyse-once / solve-many loop inside
+ *                                   sptrsv_syncfree_cuda, sptrsv_syncfree_cuda.h:456-543
+ *   sptrsv_equal_*_partition     <- equal_col_partition / equal_nnz_partition, utils.h:7-67
+ *   sptrsv_levels                <- findlevel_csc, SpTRSV_cuda/findlevel.h:71-147
+ */
+#ifndef SPTRSV_HIP_H
+#define SPTRSV_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants kept at the reference's values (SpTRSV_cuda/common.h:25-33) */
+#define SPTRSV_SUBSTITUTION_FORWARD 0
+#define SPTRSV_SUBSTITUTION_BACKWARD 1
+#define SPTRSV_OPT_WARP_NNZ 1  /* lanes over a column's nonzeros          */
+#define SPTRSV_OPT_WARP_RHS 2  /* lanes over right-hand sides             */
+#define SPTRSV_OPT_WARP_AUTO 3 /* pick per matrix (autotuner / heuristic) */
+#define SPTRSV_COL_PARTITION 0
+#define SPTRSV_NNZ_PARTITION 1
+
+/* ---- status codes.  -1 keeps the reference's meaning (m != n,
+ * sptrsv_syncfree_cuda.h:423-427); the rest are checks the reference lacks. */
+#define SPTRSV_OK 0
+#define SPTRSV_ERR_NOT_SQUARE (-1)
+#define SPTRSV_ERR_INVALID_ARG (-2)    /* null pointer, n < 0, rhs < 1, bad config */
+#define SPTRSV_ERR_MISSING_DIAG (-3)   /* diagonal not first (fwd) / last (bwd)   */
+#define SPTRSV_ERR_ZERO_DIAG (-4)
+#define SPTRSV_ERR_NOT_TRIANGULAR (-5) /* entry outside the solved triangle       */
+#define SPTRSV_ERR_UNSORTED (-6)       /* column rows not strictly ascending      */
+#define SPTRSV_ERR_INDEX (-7)          /* row index outside [0, n) / bad colPtr   */
+#define SPTRSV_ERR_HIP (-8)            /* a HIP runtime call failed               */
+#define SPTRSV_ERR_TIMEOUT (-9)        /* a spin-wait hit its cap (state reset)   */
+#define SPTRSV_ERR_NO_DEVICE (-10)
+#define SPTRSV_ERR_UNSUPPORTED (-11)   /* e.g. IR on an FP32 plan                 */
+#define SPTRSV_ERR_NO_MEMORY (-12)
+
+/* ---- value types */
+#define SPTRSV_F64 0
+#define SPTRSV_F32 1
+
+/* Human-readable name of a status code (static string). */
+const char *sptrsv_status_string(int status);
+/* Library version, e.g. "0.1.0"; the gfx target the code objects were built for. */
+const char *sptrsv_version(void);
+
+/* ========================================================================
+ * Drop-in entry points (host arrays in, host x out).
+ * Same parameter list, order and meaning as sptrsv_syncfree_cuda
+ * (SpTRSV_cuda/sptrsv_syncfree_cuda.h:409-421):
+ *   CSC of the triangular factor (0-based int32, rows ascending per column,
+ *   diagonal first for forward / last for backward), m == n, nnzTR entries;
+ *   substitution 0/1; rhs >= 1; opt 1/2/3 (matters for rhs > 1);
+ *   x, b, x_ref host arrays, row-major [n][rhs]; x is overwritten; *gflops
+ *   receives 2*rhs*nnz / mean solve time.  x_ref may be NULL (no validation).
+ * Like the reference, each call allocates and frees all device memory, runs
+ * the analysis, and times SPTRSV_BENCH_REPEAT solves (default 100, see
+ * sptrsv_set_bench_repeat).  Unlike it, the matrix is validated and errors are
+ * returned instead of hanging.  Output lines in the reference's format are
+ * printed only when sptrsv_set_verbose(1).
+ * ======================================================================== */
+int sptrsv_syncfree_hip(const int *cscColPtrTR, const int *cscRowIdxTR,
+                        const double *cscValTR, const int m, const int n,
+                        const int nnzTR, const int substitution, const int rhs,
+                        const int opt, double *x, const double *b,
+                        const double *x_ref, double *gflops);
+
+int sptrsv_syncfree_hip_f32(const int *cscColPtrTR, const int *cscRowIdxTR,
+                            const float *cscValTR, const int m, const int n,
+                            const int nnzTR, const int substitution,
+                            const int rhs, const int opt, float *x,
+                            const float *b, const float *x_ref, double *gflops);
+
+/* CSR of the triangular factor in (the reference transposes on the host at
+ * main.cu:208-210 before calling the solver); transposed on the device. */
+int sptrsv_syncfree_hip_csr(const int *csrRowPtr, const int *csrColIdx,
+                            const double *csrVal, const int m, const int n,
+                            const int nnz, const int substitution,
+                            const int rhs, const int opt, double *x,
+                            const double *b, const double *x_ref,
+                            double *gflops);
+
+void sptrsv_set_verbose(int verbose);
+void sptrsv_set_bench_repeat(int repeat); /* >= 1; BENCH_REPEAT, common.h:13-14 */
+
+/* ========================================================================
+ * Device-memory helpers (thin wrappers over the HIP runtime the library uses,
+ * so that C, ctypes, cgo, ... callers need no HIP headers).
+ * ======================================================================== */
+int sptrsv_device_count(int *count);
+int sptrsv_set_device(int device);
+int sptrsv_device_name(char *buf, int len);
+int sptrsv_device_cu_count(int *cus);
+int sptrsv_malloc(void **dptr, size_t bytes);
+int sptrsv_free(void *dptr);
+int sptrsv_memcpy_htod(void *dst, const void *src, size_t bytes);
+int sptrsv_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int sptrsv_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int sptrsv_memset(void *dptr, int value, size_t bytes);
+int sptrsv_synchronize(void);
+int sptrsv_stream_create(void **stream);
+int sptrsv_stream_destroy(void *stream);
+int sptrsv_stream_synchronize(void *stream);
+/* Event timing on a stream: returns milliseconds between two records. */
+int sptrsv_event_create(void **event);
+int sptrsv_event_destroy(void *event);
+int sptrsv_event_record(void *event, void *stream);
+int sptrsv_event_elapsed(void *start, void *stop, float *ms);
+
+/* ========================================================================
+ * Device building blocks (all arrays device-resident, asynchronous on stream
+ * unless noted).
+ * ======================================================================== */
+/* In-place exclusive scan of len ints (utils.h:374-391). */
+int sptrsv_exclusive_scan(int *d_data, int len, void *stream);
+
+/* Stable CSR -> CSC transpose of an m x n matrix with nnz entries
+ * (tranpose.h:6-43): output columns list their rows in input order, so for a
+ * CSR with ascending rows every column comes out ascending.  value_type is
+ * SPTRSV_F64/SPTRSV_F32; d_val/d_cscVal may both be NULL (pattern only).
+ * Also CSC -> CSR when given the CSC arrays (as findlevel.h:94-96 does).
+ * Synchronises the stream (the sort tiers are sized on the host). */
+int sptrsv_csr2csc(int m, int n, int nnz, const int *d_rowPtr,
+                   const int *d_colIdx, const void *d_val, int value_type,
+                   int *d_colPtr, int *d_rowIdx, void *d_cscVal, void *stream);
+
+/* in_degree[r] = number of stored entries of row r, diagonal included
+ * (sptrsv_syncfree_cuda.h:8-19). */
+int sptrsv_analyse(const int *d_cscRowIdx, int m, int nnz, int *d_inDegree,
+                   void *stream);
+
+/* Check a CSC triangular factor; returns SPTRSV_OK or the first error code
+ * (synchronises the stream). */
+int sptrsv_validate(const int *d_colPtr, const int *d_rowIdx, const void *d_val,
+                    int value_type, int n, int nnz, int substitution,
+                    void *stream);
+
+/* Host-side partitioners (utils.h:7-17, :19-67); counts/displs have nparts
+ * entries.  The nnz partitioner zero-fills counts/displs first (the reference
+ * leaves that to its callers, naive_unified.h:130-131). */
+int sptrsv_equal_col_partition(int nparts, int cols, int *counts, int *displs);
+int sptrsv_equal_nnz_partition(int nparts, int ncols, int nnz,
+                               const int *colPtr, int *counts, int *displs);
+
+/* Level-set statistics of a host CSC factor (findlevel.h:71-147). */
+int sptrsv_levels(const int *cscColPtr, const int *cscRowIdx, int n, int nnz,
+                  int *nlevel, int *pmin, int *pavg, int *pmax);
+
+/* ========================================================================
+ * Plan API: analyse once, solve many times on device-resident data.
+ * ======================================================================== */
+typedef struct sptrsv_plan_s *sptrsv_plan_t;
+
+/* Executor configuration.  lanes_per_column is the number of lanes of a
+ * 64-wide wavefront that cooperate on one column (64 = one wavefront per
+ * column, 1 = one lane per column, 64 columns per wavefront).  rhs_lanes of
+ * those lanes run over right-hand sides (1 = lanes over nonzeros,
+ * OPT_WARP_NNZ; == lanes_per_column = OPT_WARP_RHS), each lane holding
+ * rhs_per_lane right-hand sides, so one workgroup pass covers a tile of
+ * rhs_lanes*rhs_per_lane right-hand sides.  window is the number of
+ * consecutive columns whose partial sums and counters live in LDS.
+ * A zero field means "choose". */
+typedef struct {
+  int lanes_per_column; /* 1,2,4,...,64 */
+  int rhs_lanes;        /* 1 or lanes_per_column */
+  int rhs_per_lane;     /* 1..4 */
+  int block;            /* threads per workgroup: 64..1024, multiple of 64 */
+  int window;           /* LDS window, columns (max, for NNZ partition) */
+  int partition;        /* SPTRSV_COL_PARTITION / SPTRSV_NNZ_PARTITION */
+  int grid;             /* workgroups launched; 0 = resident capacity */
+} sptrsv_config_t;
+
+#define SPTRSV_PLAN_VALIDATE 1 /* check the factor at creation */
+#define SPTRSV_PLAN_MIXED 2    /* keep FP32 values + FP64 CSR for sptrsv_plan_solve_ir */
+
+/* d_colPtr/d_rowIdx/d_val are borrowed (must outlive the plan).
+ * value_type gives the type of d_val and of b/x in sptrsv_plan_solve. */
+int sptrsv_plan_create(sptrsv_plan_t *plan, int n, int nnz,
+                       const int *d_colPtr, const int *d_rowIdx,
+                       const void *d_val, int value_type, int substitution,
+                       int flags, void *stream);
+
+/* Many independent triangular systems solved by one launch: the systems are
+ * the diagonal blocks [sys_ptr[s], sys_ptr[s+1]) of one block-diagonal CSC
+ * factor (h_sys_ptr is a host array of nsys+1 column offsets, global
+ * indices).  Entries coupling two systems are rejected. */
+int sptrsv_batch_plan_create(sptrsv_plan_t *plan, int nsys,
+                             const int *h_sys_ptr, int n, int nnz,
+                             const int *d_colPtr, const int *d_rowIdx,
+                             const void *d_val, int value_type,
+                             int substitution, int flags, void *stream);
+
+int sptrsv_plan_destroy(sptrsv_plan_t plan);
+
+/* Fix the executor configuration used for `rhs` right-hand sides. */
+int sptrsv_plan_set_config(sptrsv_plan_t plan, int rhs,
+                           const sptrsv_config_t *cfg);
+/* The configuration the plan will use for `rhs` (after defaults/tuning). */
+int sptrsv_plan_get_config(sptrsv_plan_t plan, int rhs, sptrsv_config_t *cfg);
+
+/* x = L^{-1} b for rhs right-hand sides, row-major [n][rhs], in the plan's
+ * value type.  One kernel launch, no memsets (the executor restores its own
+ * counters), so it may be captured into a hipGraph. */
+int sptrsv_plan_solve(sptrsv_plan_t plan, const void *d_b, void *d_x, int rhs,
+                      void *stream);
+
+/* FP32 solves + FP64 residual/update iterative refinement (plan created with
+ * SPTRSV_PLAN_MIXED on FP64 data).  iters refinement steps after the first
+ * FP32 solve.  d_b, d_x are FP64. */
+int sptrsv_plan_solve_ir(sptrsv_plan_t plan, const double *d_b, double *d_x,
+                         int rhs, int iters, void *stream);
+
+/* Autotune: time candidate configurations on the plan's matrix and keep the
+ * fastest for `rhs` (mode 0 = plain solve, 1 = FP32 solve used inside IR).
+ * Results are cached per (n, nnz, rhs, type, depth bucket) in the file named
+ * by $SPTRSV_TUNE_CACHE when set.  Synchronous. */
+int sptrsv_plan_autotune(sptrsv_plan_t plan, int rhs, int mode,
+                         sptrsv_config_t *best, float *best_ms);
+
+/* Capture solve (mode 0) or IR solve (mode 1, `iters` steps) into a hipGraph
+ * on the given buffers; replay with sptrsv_graph_launch. */
+typedef struct sptrsv_graph_s *sptrsv_graph_t;
+int sptrsv_graph_create(sptrsv_graph_t *graph, sptrsv_plan_t plan,
+                        const void *d_b, void *d_x, int rhs, int mode,
+                        int iters, void *stream);
+int sptrsv_graph_launch(sptrsv_graph_t graph, void *stream);
+int sptrsv_graph_destroy(sptrsv_graph_t graph);
+
+/* Wait for the stream and report a spin-wait timeout of any earlier solve
+ * (SPTRSV_ERR_TIMEOUT, after which the plan's scratch state is reset). */
+int sptrsv_plan_check(sptrsv_plan_t plan, void *stream);
+/* Clear the plan's scratch state (counters, partial sums, tickets). */
+int sptrsv_plan_reset(sptrsv_plan_t plan, void *stream);
+/* Cap on polls of one spin-wait before a solve gives up (0 = default). */
+int sptrsv_plan_set_spin_cap(sptrsv_plan_t plan, long long cap);
+
+typedef struct {
+  int n, nnz, substitution, value_type, nsys;
+  int levels;                /* DAG depth (0 if not computed)          */
+  long long scratch_bytes;   /* device scratch held by the plan        */
+  double analysis_ms;        /* device analysis time at creation       */
+} sptrsv_plan_info_t;
+int sptrsv_plan_info(sptrsv_plan_t plan, sptrsv_plan_info_t *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPTRSV_HIP_H */
