@@ -111,31 +111,43 @@ def power_law_lower(n, s=2.0, cap=1000, window=50000, seed=0):
     Row i draws k_i = min(Zipf(s), cap, w_i) off-diagonal columns, w_i = i -
     max(0, i - window), uniformly without replacement from [i - w_i, i);
     values U(-1, 1); diagonal 1 + sum |row|.  Draw order with
-    numpy.random.default_rng(seed): all k (zipf), the columns (integers, with
-    duplicate redraws), then the values (uniform).  The x vector is drawn by
-    the caller.
+    numpy.random.default_rng(seed): all k (zipf); the columns of rows with
+    2 k_i <= w_i (vectorised integers, duplicates redrawn until none remain);
+    the columns of the remaining rows, one rng.choice(w_i, k_i, replace=False)
+    per row in ascending row order; then the values (uniform).  The x vector is
+    drawn by the caller.
     """
     rng = np.random.default_rng(seed)
     i = np.arange(n, dtype=np.int64)
     w = i - np.maximum(0, i - window)
     k = np.minimum(np.minimum(rng.zipf(s, size=n), cap), w).astype(np.int64)
-    rows = np.repeat(i, k)
-    lo = np.repeat(i - w, k)
-    span = np.repeat(w, k)
+    dense = 2 * k > w
+    ks = np.where(dense, 0, k)
+    rows = np.repeat(i, ks)
+    lo = np.repeat(i - w, ks)
+    span = np.repeat(w, ks)
     cols = lo + rng.integers(0, np.maximum(span, 1))
     # redraw duplicates within a row until every row's columns are distinct
-    for _ in range(1000):
+    for _ in range(200):
         key = rows * (n + 1) + cols
         order = np.argsort(key, kind="stable")
-        ks = key[order]
-        dup = np.zeros(len(ks), bool)
-        dup[1:] = ks[1:] == ks[:-1]
+        kk = key[order]
+        dup = np.zeros(len(kk), bool)
+        dup[1:] = kk[1:] == kk[:-1]
         if not dup.any():
             break
         idx = order[dup]
         cols[idx] = lo[idx] + rng.integers(0, span[idx])
     else:
         raise RuntimeError("power_law_lower: duplicate redraw did not converge")
+    drows, dcols = [], []
+    for r in np.flatnonzero(dense):
+        pick = rng.choice(int(w[r]), size=int(k[r]), replace=False)
+        drows.append(np.full(len(pick), r, np.int64))
+        dcols.append(r - w[r] + pick.astype(np.int64))
+    if drows:
+        rows = np.concatenate([rows] + drows)
+        cols = np.concatenate([cols] + dcols)
     vals = rng.uniform(-1.0, 1.0, size=len(rows))
     absum = np.bincount(rows, weights=np.abs(vals), minlength=n)
     diag = 1.0 + absum
