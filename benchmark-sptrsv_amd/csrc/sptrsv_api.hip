@@ -1152,7 +1152,10 @@ int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg)
   sptrsv_config_t c = *cfg;
   const sptrsv_config_t d = default_config(p, rhs);
   if (c.lanes_per_column == 0) c.lanes_per_column = d.lanes_per_column;
-  if (c.rhs_lanes == 0) c.rhs_lanes = std::min(d.rhs_lanes, c.lanes_per_column);
+  // lanes over right-hand sides when the group is no wider than the default
+  // rhs lane count, otherwise lanes over nonzeros
+  if (c.rhs_lanes == 0)
+    c.rhs_lanes = d.rhs_lanes >= c.lanes_per_column ? c.lanes_per_column : 1;
   if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
   if (c.block == 0) c.block = d.block;
   if (c.window == 0) c.window = c.block / c.lanes_per_column;
