@@ -15,6 +15,7 @@
 #include "../../include/sptrsv_hip.h"
 #include "sptrsv_exec.hpp"
 #include "sptrsv_ir.hpp"
+#include "sptrsv_lane.hpp"
 #include "sptrsv_transpose.hpp"
 
 using namespace sptrsv;
@@ -366,6 +367,10 @@ struct WindowSet {
   DevBuf need;    // int2[n]
   int nwin = 0;
   int maxw = 0;
+  // lane engine: slot ranges per row and slot targets per entry
+  DevBuf rptr;    // int[n+1]
+  DevBuf etgt;    // int[nnz]
+  int nslots = -1;
 };
 
 struct sptrsv_plan_s {
@@ -376,6 +381,7 @@ struct sptrsv_plan_s {
   DevBuf inDegree;
   DevBuf sync;  // int[4]: ticket, finished, err, pad
   DevBuf gsum, gcnt;
+  DevBuf slots64, slots32;  // lane-engine slots, kept at the empty sentinel
   std::map<long long, WindowSet *> windows;     // key: window*2 + partition
   std::map<long long, sptrsv_config_t> configs;  // key: rhs*2 + is_f32_solve
   // mixed precision
@@ -415,6 +421,19 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
 }
 
 int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
+  if (c.engine == SPTRSV_ENGINE_LANE) {
+    if (c.lanes_per_column != 1 || c.rhs_lanes != 1 || c.rhs_per_lane != 1)
+      return SPTRSV_ERR_INVALID_ARG;
+    if (c.block < 128 || c.block > 1024 || c.block % 64 != 0 || c.window < 1)
+      return SPTRSV_ERR_INVALID_ARG;
+    if (c.partition != SPTRSV_COL_PARTITION &&
+        c.partition != SPTRSV_NNZ_PARTITION)
+      return SPTRSV_ERR_INVALID_ARG;
+    if ((size_t)c.window * (tsize + sizeof(int)) > p->lds_max)
+      return SPTRSV_ERR_INVALID_ARG;
+    return SPTRSV_OK;
+  }
+  if (c.engine != SPTRSV_ENGINE_GROUP) return SPTRSV_ERR_INVALID_ARG;
   if (!is_pow2(c.lanes_per_column) || c.lanes_per_column > 64)
     return SPTRSV_ERR_INVALID_ARG;
   if (!(c.rhs_lanes == 1 || c.rhs_lanes == c.lanes_per_column))
@@ -566,6 +585,61 @@ int get_windows(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet **out,
   return SPTRSV_OK;
 }
 
+template <typename T>
+T slot_empty_host() {
+  T v;
+  typename SlotBits<T>::U u = SlotBits<T>::kEmpty;
+  memcpy(&v, &u, sizeof(T));
+  return v;
+}
+
+// Lane-engine analysis of a window partition: slot ranges per row (exclusive
+// scan of the remote arrival counts) and the slot or LDS target of every
+// off-diagonal entry.
+int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
+                     hipStream_t s) {
+  if (ws->nslots >= 0) return SPTRSV_OK;
+  const int n = p->n;
+  ST_TRY(ws->rptr.ensure(sizeof(int) * ((size_t)n + 1), false));
+  ST_TRY(ws->etgt.ensure(sizeof(int) * (size_t)std::max(p->nnz, 1), false));
+  if (n > 0) {
+    k_need_remote<<<grid_for(n, 256), 256, 0, s>>>(ws->need.as<int2>(), n,
+                                                   ws->rptr.as<int>());
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemsetAsync(ws->rptr.as<int>() + n, 0, sizeof(int), s));
+  ST_TRY(scan_rec(ws->rptr.as<int>(), n + 1, s));
+  int nslots = 0;
+  HIP_TRY(hipMemcpyAsync(&nslots, ws->rptr.as<int>() + n, sizeof(int),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n > 0) {
+    DevBuf cursor;
+    ST_TRY(cursor.ensure(sizeof(int) * (size_t)n, true));
+    k_lane_targets<<<grid_for(n, 256), 256, 0, s>>>(
+        p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+        ws->winPtr.as<int>(), ws->nwin,
+        c.partition == SPTRSV_COL_PARTITION ? c.window : 0, ws->rptr.as<int>(),
+        cursor.as<int>(), ws->etgt.as<int>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cursor.release();
+    if (e != hipSuccess) return SPTRSV_ERR_HIP;
+  }
+  ws->nslots = nslots;
+  return SPTRSV_OK;
+}
+
+template <typename T>
+DevBuf &slot_buf(sptrsv_plan_s *p);
+template <>
+DevBuf &slot_buf<double>(sptrsv_plan_s *p) { return p->slots64; }
+template <>
+DevBuf &slot_buf<float>(sptrsv_plan_s *p) { return p->slots32; }
+
+template <typename T>
+using LaneKernel = void (*)(LaneArgs<T>);
+
 // Everything a solve launch needs, prepared outside any graph capture.
 struct Launch {
   sptrsv_config_t cfg;
@@ -583,6 +657,37 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
   const sptrsv_config_t &c = L->cfg;
   ST_TRY(check_config(p, c, sizeof(T)));
   ST_TRY(get_windows(p, c, &L->ws, s));
+  if (c.engine == SPTRSV_ENGINE_LANE) {
+    ST_TRY(get_lane_targets(p, c, L->ws, s));
+    L->R = 1;
+    L->ntiles = rhs;
+    L->lds = (size_t)L->ws->maxw * (sizeof(T) + sizeof(int));
+    DevBuf &sb = slot_buf<T>(p);
+    const size_t want =
+        std::max<size_t>(1, (size_t)L->ntiles * L->ws->nslots) * sizeof(T);
+    if (want > sb.bytes) {
+      ST_TRY(sb.ensure(want, false));
+      ST_TRY(fill_async<T>(sb.as<T>(), sb.bytes / sizeof(T),
+                           slot_empty_host<T>(), s));
+    }
+    LaneKernel<T> kern = p->subst == SPTRSV_SUBSTITUTION_BACKWARD
+                             ? k_exec_lane<T, true>
+                             : k_exec_lane<T, false>;
+    if (L->lds > 65536)
+      HIP_TRY(hipFuncSetAttribute((const void *)kern,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)L->lds));
+    const int total = L->ws->nwin * L->ntiles;
+    int grid = c.grid;
+    if (grid <= 0) {
+      int per_cu = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern,
+                                                           c.block, L->lds));
+      grid = std::max(1, per_cu) * p->cus;
+    }
+    L->grid = std::max(1, std::min(grid, std::max(total, 1)));
+    return SPTRSV_OK;
+  }
   L->R = c.rhs_lanes * c.rhs_per_lane;
   L->ntiles = (rhs + L->R - 1) / L->R;
   L->lds = (size_t)L->ws->maxw * ((size_t)L->R * sizeof(T) + sizeof(int));
@@ -617,6 +722,33 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
                 T *x, int rhs, hipStream_t s) {
   if (p->n == 0) return SPTRSV_OK;
   const sptrsv_config_t &c = L.cfg;
+  if (c.engine == SPTRSV_ENGINE_LANE) {
+    LaneArgs<T> la;
+    la.colPtr = p->colPtr;
+    la.val = val;
+    la.etgt = L.ws->etgt.as<int>();
+    la.need = L.ws->need.as<int2>();
+    la.rptr = L.ws->rptr.as<int>();
+    la.winPtr = L.ws->winPtr.as<int>();
+    la.slots = slot_buf<T>(p).template as<T>();
+    la.b = b;
+    la.x = x;
+    la.n = p->n;
+    la.rhs = rhs;
+    la.nwin = L.ws->nwin;
+    la.ntiles = L.ntiles;
+    la.maxw = L.ws->maxw;
+    la.nslots = L.ws->nslots;
+    la.ticket = p->sync.as<int>();
+    la.err = p->sync.as<int>() + 2;
+    la.spin_cap = p->spin_cap;
+    LaneKernel<T> kern = p->subst == SPTRSV_SUBSTITUTION_BACKWARD
+                             ? k_exec_lane<T, true>
+                             : k_exec_lane<T, false>;
+    kern<<<L.grid, c.block, L.lds, s>>>(la);
+    HIP_TRY(hipGetLastError());
+    return SPTRSV_OK;
+  }
   ExecArgs<T> a;
   a.colPtr = p->colPtr;
   a.rowIdx = p->rowIdx;
@@ -645,6 +777,12 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
 }
 
 int reset_state(sptrsv_plan_s *p, hipStream_t s) {
+  if (p->slots64.p)
+    ST_TRY(fill_async<double>(p->slots64.as<double>(), p->slots64.bytes / 8,
+                              slot_empty_host<double>(), s));
+  if (p->slots32.p)
+    ST_TRY(fill_async<float>(p->slots32.as<float>(), p->slots32.bytes / 4,
+                             slot_empty_host<float>(), s));
   if (p->gsum.p) HIP_TRY(hipMemsetAsync(p->gsum.p, 0, p->gsum.bytes, s));
   if (p->gcnt.p) HIP_TRY(hipMemsetAsync(p->gcnt.p, 0, p->gcnt.bytes, s));
   HIP_TRY(hipMemsetAsync(p->sync.p, 0, p->sync.bytes, s));
@@ -788,9 +926,9 @@ bool tune_cache_lookup(const std::string &key, sptrsv_config_t *c) {
     if (key != line) continue;
     sptrsv_config_t t;
     memset(&t, 0, sizeof(t));
-    if (sscanf(bar + 1, "%d %d %d %d %d %d %d", &t.lanes_per_column,
+    if (sscanf(bar + 1, "%d %d %d %d %d %d %d %d", &t.lanes_per_column,
                &t.rhs_lanes, &t.rhs_per_lane, &t.block, &t.window,
-               &t.partition, &t.grid) == 7) {
+               &t.partition, &t.grid, &t.engine) == 8) {
       *c = t;
       found = true;  // last entry wins
     }
@@ -804,17 +942,20 @@ void tune_cache_store(const std::string &key, const sptrsv_config_t &c) {
   if (!path || !*path) return;
   FILE *f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s|%d %d %d %d %d %d %d\n", key.c_str(), c.lanes_per_column,
-          c.rhs_lanes, c.rhs_per_lane, c.block, c.window, c.partition, c.grid);
+  fprintf(f, "%s|%d %d %d %d %d %d %d %d\n", key.c_str(), c.lanes_per_column,
+          c.rhs_lanes, c.rhs_per_lane, c.block, c.window, c.partition, c.grid,
+          c.engine);
   fclose(f);
 }
 
 std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
                                              int tsize) {
   std::vector<sptrsv_config_t> out;
-  auto add = [&](int G, int RL, int rpl, int block, int window, int part) {
+  auto add = [&](int G, int RL, int rpl, int block, int window, int part,
+                 int engine = SPTRSV_ENGINE_GROUP) {
     sptrsv_config_t c;
     memset(&c, 0, sizeof(c));
+    c.engine = engine;
     c.lanes_per_column = G;
     c.rhs_lanes = RL;
     c.rhs_per_lane = rpl;
@@ -823,6 +964,9 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     c.partition = part;
     if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
   };
+  for (int block : {256, 512, 1024})
+    for (int window : {1024, 2048, 4096, 8192})
+      add(1, 1, 1, block, window, 0, SPTRSV_ENGINE_LANE);
   if (rhs <= 1) {
     for (int G : {1, 2, 4, 8, 16, 64})
       for (int block : {256, 1024}) {
@@ -1136,11 +1280,14 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
   for (auto &kv : p->windows) {
     kv.second->winPtr.release();
     kv.second->need.release();
+    kv.second->rptr.release();
+    kv.second->etgt.release();
     delete kv.second;
   }
   p->windows.clear();
-  DevBuf *bufs[] = {&p->inDegree, &p->sync, &p->gsum,   &p->gcnt, &p->val32,
-                    &p->csrPtr,   &p->csrIdx, &p->csrVal, &p->xtmp, &p->r32,
+  DevBuf *bufs[] = {&p->inDegree, &p->sync,    &p->gsum,    &p->gcnt,
+                    &p->slots64,  &p->slots32, &p->val32,   &p->csrPtr,
+                    &p->csrIdx,   &p->csrVal,  &p->xtmp,    &p->r32,
                     &p->d32};
   for (DevBuf *b : bufs) b->release();
   delete p;
@@ -1151,6 +1298,13 @@ int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg)
   if (!p || !cfg || rhs < 1) return SPTRSV_ERR_INVALID_ARG;
   sptrsv_config_t c = *cfg;
   const sptrsv_config_t d = default_config(p, rhs);
+  if (c.engine == SPTRSV_ENGINE_LANE) {
+    if (c.lanes_per_column == 0) c.lanes_per_column = 1;
+    if (c.rhs_lanes == 0) c.rhs_lanes = 1;
+    if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
+    if (c.block == 0) c.block = 1024;
+    if (c.window == 0) c.window = 4096;
+  }
   if (c.lanes_per_column == 0) c.lanes_per_column = d.lanes_per_column;
   // lanes over right-hand sides when the group is no wider than the default
   // rhs lane count, otherwise lanes over nonzeros
@@ -1235,12 +1389,14 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
   info->value_type = p->vt;
   info->nsys = p->nsys;
   long long bytes = 0;
-  const DevBuf *bufs[] = {&p->inDegree, &p->sync, &p->gsum, &p->gcnt,
-                          &p->val32,    &p->csrPtr, &p->csrIdx, &p->csrVal,
-                          &p->xtmp,     &p->r32,  &p->d32};
+  const DevBuf *bufs[] = {&p->inDegree, &p->sync,    &p->gsum,  &p->gcnt,
+                          &p->slots64,  &p->slots32, &p->val32, &p->csrPtr,
+                          &p->csrIdx,   &p->csrVal,  &p->xtmp,  &p->r32,
+                          &p->d32};
   for (const DevBuf *b : bufs) bytes += (long long)b->bytes;
   for (auto &kv : p->windows)
-    bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes);
+    bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes +
+                         kv.second->rptr.bytes + kv.second->etgt.bytes);
   info->scratch_bytes = bytes;
   info->analysis_ms = p->analysis_ms;
   return SPTRSV_OK;
@@ -1295,8 +1451,8 @@ int sptrsv_plan_autotune(sptrsv_plan_t p, int rhs, int mode,
       bc = c;
     }
     if (g_verbose)
-      fprintf(stderr, "tune G=%d RL=%d rpl=%d block=%d window=%d : %s %.4f ms\n",
-              c.lanes_per_column, c.rhs_lanes, c.rhs_per_lane, c.block,
+      fprintf(stderr, "tune engine=%d G=%d RL=%d rpl=%d block=%d window=%d : %s %.4f ms\n",
+              c.engine, c.lanes_per_column, c.rhs_lanes, c.rhs_per_lane, c.block,
               c.window, sptrsv_status_string(r), ms);
   }
   bb.release();

@@ -1,0 +1,287 @@
+// sptrsv_lane.hpp -- the lane engine: one lane per column, LDS windows, and
+// value-carrying slots for dependencies between windows.
+//
+// Same solve semantics as the wavefront-group executor (sptrsv_exec.hpp) and
+// the serial executor it restates (sptrsv_syncfree_serialref.h:25-108), with
+// a memory protocol shaped for gfx950's counters:
+//
+//  * A workgroup owns a window of consecutive columns (solve order), claimed
+//    through the global ticket in increasing order.  Its LDS holds one partial
+//    value per column, initialised with b, from which every contribution
+//    L(r, c) * x_c is subtracted, plus one arrival counter per column.
+//  * Compute wavefronts poll and update LDS only; their loop never waits on a
+//    vector-memory access (gfx9's vmcnt counts stores too, so one outstanding
+//    store or load would otherwise stall every poll of the wavefront).
+//    Column metadata and the first kPF off-diagonal entries of 64 columns are
+//    loaded into registers once per batch.
+//  * A contribution to a row of a later window is stored into that entry's own
+//    slot in HBM.  The slot holds a NaN sentinel while empty, so the stored
+//    value is its own arrival flag: one relaxed store by the producer, one
+//    relaxed load by the consumer, no atomic read-modify-write, no fence and
+//    no counter.  Slots are grouped by row (rptr) and summed in slot order.
+//  * Wavefront 0 of each workgroup is the receiver: it polls the slots of the
+//    window's rows, folds complete sums into LDS and resets the slots to the
+//    sentinel, so every solve leaves the slots ready for the next one.
+#pragma once
+
+#include "sptrsv_device.hpp"
+
+namespace sptrsv {
+
+template <typename T>
+struct SlotBits;
+template <>
+struct SlotBits<double> {
+  using U = unsigned long long;
+  static constexpr U kEmpty = 0x7ff4deadbeef5afeull;  // signalling-NaN payload
+};
+template <>
+struct SlotBits<float> {
+  using U = unsigned int;
+  static constexpr U kEmpty = 0x7fa5afe5u;
+};
+
+template <typename T>
+__device__ __forceinline__ bool slot_empty(T v) {
+  return __builtin_bit_cast(typename SlotBits<T>::U, v) == SlotBits<T>::kEmpty;
+}
+template <typename T>
+__device__ __forceinline__ T slot_empty_value() {
+  return __builtin_bit_cast(T, SlotBits<T>::kEmpty);
+}
+
+template <typename T>
+struct LaneArgs {
+  const int *__restrict__ colPtr;
+  const T *__restrict__ val;
+  const int *__restrict__ etgt;   // [nnz]: local -(lk+1) or slot index
+  const int2 *__restrict__ need;  // [n] by row: {local, remote} arrivals
+  const int *__restrict__ rptr;   // [n+1] slot range of each row
+  const int *__restrict__ winPtr; // [nwin + 1] window bounds, solve order
+  T *slots;                       // [ntiles][nslots]
+  const T *__restrict__ b;        // [n][rhs]
+  T *__restrict__ x;              // [n][rhs]
+  int n, rhs, nwin, ntiles, maxw, nslots;
+  int *ticket;
+  int *err;
+  long long spin_cap;
+};
+
+constexpr int kPF = 4;  // off-diagonal entries per column held in registers
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
+                                            int *s_cnt, int wlo, int wlen,
+                                            int tile, int cw, int ncw) {
+  const int lane = __lane_id();
+  const int n = a.n;
+  T *slots = a.slots + (size_t)tile * a.nslots;
+  for (int base = cw * kWave; base < wlen; base += ncw * kWave) {
+    const int k = base + lane;
+    bool active = k < wlen;
+    int lo = 0, hi = 0, total = 0;
+    T diag = T(1);
+    int tg[kPF];
+    T vv[kPF];
+    if (active) {
+      const int p = wlo + k;
+      const int c = BWD ? n - 1 - p : p;
+      const int f = a.colPtr[c], l = a.colPtr[c + 1];
+      diag = a.val[BWD ? l - 1 : f];
+      lo = BWD ? f : f + 1;
+      hi = BWD ? l - 1 : l;
+      const int2 nd = a.need[c];
+      total = nd.x + nd.y;
+#pragma unroll
+      for (int i = 0; i < kPF; i++) {
+        const int j = lo + i;
+        tg[i] = j < hi ? a.etgt[j] : -1;
+        vv[i] = j < hi ? a.val[j] : T(0);
+      }
+    }
+    long long spins = 0;
+    while (__ballot(active) != 0) {
+      if (active && ld_wg(&s_cnt[k]) == total) {
+        fence_acq_wg();
+        const T xv = s_val[k] / diag;
+        s_val[k] = xv;  // the column is final: its slot now holds x
+#pragma unroll
+        for (int i = 0; i < kPF; i++)
+          if (lo + i < hi && tg[i] < 0) add_wg(&s_val[-tg[i] - 1], -(xv * vv[i]));
+        for (int j = lo + kPF; j < hi; j++) {
+          const int t = a.etgt[j];
+          if (t < 0) add_wg(&s_val[-t - 1], -(xv * a.val[j]));
+        }
+        fence_rel_wg();
+#pragma unroll
+        for (int i = 0; i < kPF; i++) {
+          if (lo + i < hi) {
+            if (tg[i] < 0)
+              add_wg(&s_cnt[-tg[i] - 1], 1);
+            else
+              st_agent(&slots[tg[i]], xv * vv[i]);
+          }
+        }
+        for (int j = lo + kPF; j < hi; j++) {
+          const int t = a.etgt[j];
+          if (t < 0)
+            add_wg(&s_cnt[-t - 1], 1);
+          else
+            st_agent(&slots[t], xv * a.val[j]);
+        }
+        active = false;
+      } else if (active) {
+        ++spins;
+        if ((spins & 4095) == 0 && ld_agent(a.err) != 0) active = false;
+        if (spins > a.spin_cap) {
+          st_agent(a.err, 1);
+          active = false;
+        }
+      }
+    }
+  }
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
+                                            int *s_cnt, int wlo, int wlen,
+                                            int tile) {
+  const int lane = __lane_id();
+  const int n = a.n;
+  T *slots = a.slots + (size_t)tile * a.nslots;
+  int k = lane;
+  int r0 = 0, r1 = 0;
+  auto load = [&]() {
+    const int p = wlo + k;
+    const int c = BWD ? n - 1 - p : p;
+    r0 = a.rptr[c];
+    r1 = a.rptr[c + 1];
+  };
+  bool active = k < wlen;
+  if (active) load();
+  long long spins = 0;
+  while (__ballot(active) != 0) {
+    bool done = r0 == r1;
+    if (active && !done) {
+      T sum = T(0);
+      bool full = true;
+      for (int s = r0; s < r1 && full; s += 4) {
+        T v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          v[q] = s + q < r1 ? ld_agent(&slots[s + q]) : T(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (s + q < r1) {
+            if (slot_empty(v[q]))
+              full = false;
+            else
+              sum += v[q];
+          }
+        }
+      }
+      if (full) {
+        for (int s = r0; s < r1; s++) st_agent(&slots[s], slot_empty_value<T>());
+        add_wg(&s_val[k], -sum);
+        fence_rel_wg();
+        add_wg(&s_cnt[k], r1 - r0);
+        done = true;
+      } else {
+        ++spins;
+        if ((spins & 1023) == 0 && ld_agent(a.err) != 0) active = false;
+        if (spins > a.spin_cap) {
+          st_agent(a.err, 1);
+          active = false;
+        }
+      }
+    }
+    if (done && active) {
+      spins = 0;
+      k += kWave;
+      active = k < wlen;
+      if (active) load();
+    }
+  }
+}
+
+template <typename T, bool BWD>
+__global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *s_val = reinterpret_cast<T *>(smem_raw);
+  int *s_cnt = reinterpret_cast<int *>(s_val + a.maxw);
+  __shared__ int s_ticket;
+  const int total = a.nwin * a.ntiles;
+  const int wave = threadIdx.x / kWave;
+  const int ncw = blockDim.x / kWave - 1;
+  const int n = a.n;
+
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_ticket = ld_agent(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
+    __syncthreads();
+    const int t = s_ticket;
+    if (t >= total) break;
+    const int win = t / a.ntiles;
+    const int tile = t - win * a.ntiles;
+    const int wlo = a.winPtr[win];
+    const int wlen = a.winPtr[win + 1] - wlo;
+    for (int k = threadIdx.x; k < wlen; k += blockDim.x) {
+      const int p = wlo + k;
+      const int c = BWD ? n - 1 - p : p;
+      s_val[k] = a.b[(size_t)c * a.rhs + tile];
+      s_cnt[k] = 0;
+    }
+    __syncthreads();
+    if (wave == 0)
+      lane_receive<T, BWD>(a, s_val, s_cnt, wlo, wlen, tile);
+    else
+      lane_compute<T, BWD>(a, s_val, s_cnt, wlo, wlen, tile, wave - 1, ncw);
+    __syncthreads();
+    for (int k = threadIdx.x; k < wlen; k += blockDim.x) {
+      const int p = wlo + k;
+      const int c = BWD ? n - 1 - p : p;
+      a.x[(size_t)c * a.rhs + tile] = s_val[k];
+    }
+  }
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(a.ticket + 1, 1);
+    if (done == (int)gridDim.x - 1) {
+      st_agent(a.ticket, 0);
+      st_agent(a.ticket + 1, 0);
+    }
+  }
+}
+
+// Slot targets of the off-diagonal entries for a window partition: an entry
+// whose row lies in the column's window gets -(row's window offset + 1); any
+// other entry gets the next slot of its row (cursor zeroed, rptr = exclusive
+// scan of need.y).
+__global__ void k_lane_targets(const int *__restrict__ colPtr,
+                               const int *__restrict__ rowIdx, int n,
+                               int backward, const int *__restrict__ winPtr,
+                               int nwin, int uniform_w,
+                               const int *__restrict__ rptr, int *cursor,
+                               int *etgt) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += gridDim.x * blockDim.x) {
+    const int p = backward ? n - 1 - c : c;
+    const int w = uniform_w > 0 ? p / uniform_w
+                                : upper_bound_minus1(winPtr, nwin + 1, p);
+    const int wlo = winPtr[w], whi = winPtr[w + 1];
+    const int f = colPtr[c], l = colPtr[c + 1];
+    const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
+    for (int j = lo; j < hi; j++) {
+      const int r = rowIdx[j];
+      const int pr = backward ? n - 1 - r : r;
+      etgt[j] = pr < whi ? -(pr - wlo) - 1 : rptr[r] + atomicAdd(&cursor[r], 1);
+    }
+  }
+}
+
+__global__ void k_need_remote(const int2 *__restrict__ need, int n, int *out) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += gridDim.x * blockDim.x)
+    out[r] = need[r].y;
+}
+
+}  // namespace sptrsv

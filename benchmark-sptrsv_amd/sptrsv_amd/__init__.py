@@ -11,14 +11,15 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (COL_PARTITION, EXPORTS, F32, F64, LIB_PATH, NNZ_PARTITION,
+from ._lib import (COL_PARTITION, ENGINE_GROUP, ENGINE_LANE, EXPORTS, F32, F64,
+                   LIB_PATH, NNZ_PARTITION,
                    OPT_WARP_AUTO, OPT_WARP_NNZ, OPT_WARP_RHS, PLAN_MIXED,
                    PLAN_VALIDATE, STATUS, SUBSTITUTION_BACKWARD,
                    SUBSTITUTION_FORWARD, Config, PlanInfo, SptrsvError, check,
                    lib)
 
 __all__ = [
-    "COL_PARTITION", "NNZ_PARTITION", "OPT_WARP_AUTO", "OPT_WARP_NNZ",
+    "COL_PARTITION", "NNZ_PARTITION", "ENGINE_GROUP", "ENGINE_LANE", "OPT_WARP_AUTO", "OPT_WARP_NNZ",
     "OPT_WARP_RHS", "SUBSTITUTION_BACKWARD", "SUBSTITUTION_FORWARD", "F32",
     "F64", "PLAN_MIXED", "PLAN_VALIDATE", "STATUS", "EXPORTS", "LIB_PATH",
     "Config", "SptrsvError", "lib", "check", "DeviceArray", "Plan", "Graph",

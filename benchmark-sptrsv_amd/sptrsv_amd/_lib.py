@@ -21,6 +21,8 @@ F64 = 0
 F32 = 1
 PLAN_VALIDATE = 1
 PLAN_MIXED = 2
+ENGINE_GROUP = 0
+ENGINE_LANE = 1
 
 STATUS = {
     0: "OK",
@@ -75,6 +77,7 @@ class Config(ctypes.Structure):
         ("window", ctypes.c_int),
         ("partition", ctypes.c_int),
         ("grid", ctypes.c_int),
+        ("engine", ctypes.c_int),
     ]
 
     def as_dict(self):

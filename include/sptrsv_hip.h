@@ -187,7 +187,17 @@ typedef struct {
   int window;           /* LDS window, columns (max, for NNZ partition) */
   int partition;        /* SPTRSV_COL_PARTITION / SPTRSV_NNZ_PARTITION */
   int grid;             /* workgroups launched; 0 = resident capacity */
+  int engine;           /* SPTRSV_ENGINE_GROUP or SPTRSV_ENGINE_LANE */
 } sptrsv_config_t;
+
+/* Executor engines.  GROUP: lane groups of a wavefront per column, counters
+ * and partial sums in LDS within a window and in HBM (atomics + fences)
+ * across windows -- all rhs modes.  LANE: one lane per column, compute
+ * wavefronts that touch only LDS, and one value-carrying slot per
+ * cross-window entry polled by a receiver wavefront -- one right-hand side per
+ * tile (lanes_per_column = rhs_lanes = rhs_per_lane = 1, block >= 128). */
+#define SPTRSV_ENGINE_GROUP 0
+#define SPTRSV_ENGINE_LANE 1
 
 #define SPTRSV_PLAN_VALIDATE 1 /* check the factor at creation */
 #define SPTRSV_PLAN_MIXED 2    /* keep FP32 values + FP64 CSR for sptrsv_plan_solve_ir */

@@ -46,8 +46,10 @@ def test_library_reproduces_golden(path):
     g = load(path)
     rhs, sub = int(g["rhs"]), int(g["substitution"])
     f = mx.Factor(g["colPtr"], g["rowIdx"], g["val"])
+    import sptrsv_amd as S
     for cfg in (None, dict(lanes_per_column=1, block=256, window=64),
-                dict(lanes_per_column=64, block=1024, window=16)):
+                dict(lanes_per_column=64, block=1024, window=16),
+                dict(engine=S.ENGINE_LANE, block=256, window=100)):
         x, _ = plan_solve(f, g["b"], rhs=rhs, subst=sub, cfg=cfg)
         assert rel_inf(x, g["x"]) <= 1e-12
         if "intunit" in path:

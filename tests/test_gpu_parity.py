@@ -105,12 +105,17 @@ RHS1_CONFIGS = [None] + [
     dict(lanes_per_column=g, rhs_lanes=1, rhs_per_lane=1, block=blk, window=(blk // g) * mult)
     for g in (1, 2, 4, 8, 16, 32, 64) for blk, mult in ((256, 1), (256, 4), (1024, 1))
 ] + [dict(lanes_per_column=4, rhs_lanes=1, rhs_per_lane=1, block=256, window=256,
-          partition=S.NNZ_PARTITION)]
+          partition=S.NNZ_PARTITION)] + [
+    dict(engine=S.ENGINE_LANE, block=blk, window=w, partition=part)
+    for blk, w, part in ((128, 64, 0), (256, 1000, 0), (1024, 4096, 0), (1024, 8192, 0),
+                         (512, 2048, S.NNZ_PARTITION))]
 
 
 def _cfg_id(c):
     if c is None:
         return "default"
+    if c.get("engine") == S.ENGINE_LANE:
+        return "lane-B{block}-W{window}".format(**c) + ("-nnz" if c.get("partition") else "")
     return "G{lanes_per_column}-B{block}-W{window}".format(**c) + (
         "-nnz" if c.get("partition") else "")
 
@@ -144,7 +149,8 @@ def test_power_law_matches_oracle(cfg):
 def test_backward_on_transposed_laplacian():
     f = mx.upper_from_lower(mx.laplacian3d_lower(20, 17, 9))
     b = f.spmv(mx.x_uniform(f.n, seed=3))
-    for cfg in (None, dict(lanes_per_column=1, block=256, window=1024)):
+    for cfg in (None, dict(lanes_per_column=1, block=256, window=1024),
+                dict(engine=S.ENGINE_LANE, block=512, window=700)):
         x, _ = plan_solve(f, b, subst=1, cfg=cfg)
         assert_close(x, oracle_x(f, b, subst=1), 1e-12)
 
@@ -156,17 +162,30 @@ def test_repeated_solves_restore_state():
     ref = None
     for k in range(6):
         b = f.spmv(mx.x_uniform(f.n, seed=k))
-        cfg = dict(lanes_per_column=[1, 4, 64][k % 3], block=256)
+        cfg = [dict(lanes_per_column=1, block=256), dict(engine=S.ENGINE_LANE, block=256),
+               dict(lanes_per_column=64, block=256)][k % 3]
         x, _ = plan_solve(f, b, cfg=cfg, plan=plan)
         assert_close(x, oracle_x(f, b), 1e-12)
 
 
-def test_fp32_plan():
+@pytest.mark.parametrize("cfg", [None, dict(engine=S.ENGINE_LANE, block=1024, window=4096)],
+                         ids=["default", "lane"])
+def test_fp32_plan(cfg):
     f = mx.power_law_lower(20_000, window=2000, seed=5)
     b = f.spmv(mx.x_uniform(f.n, seed=1))
-    x, _ = plan_solve(f, b, dtype=np.float32)
+    x, _ = plan_solve(f, b, dtype=np.float32, cfg=cfg)
     x_o = oracle_x(f, b, dtype=np.float32)
     assert_close(x, x_o, 2e-5)
+
+
+def test_lane_engine_long_columns_overflow_registers():
+    # columns with many more off-diagonal entries than the register prefetch
+    f = mx.power_law_lower(20_000, window=400, seed=12)
+    assert np.diff(f.colPtr).max() > 12
+    b = f.spmv(mx.x_uniform(f.n, seed=2))
+    for w in (64, 2048):
+        x, _ = plan_solve(f, b, cfg=dict(engine=S.ENGINE_LANE, block=256, window=w))
+        assert_close(x, oracle_x(f, b), 1e-12)
 
 
 # ------------------------------------------------------------ multi-RHS
@@ -176,12 +195,20 @@ RHS_CONFIGS = [None,
                dict(lanes_per_column=16, rhs_lanes=16, rhs_per_lane=4, block=256, window=64),
                dict(lanes_per_column=64, rhs_lanes=1, rhs_per_lane=4, block=256, window=4),
                dict(lanes_per_column=1, rhs_lanes=1, rhs_per_lane=1, block=256, window=256),
-               dict(lanes_per_column=4, rhs_lanes=1, rhs_per_lane=4, block=256, window=64)]
+               dict(lanes_per_column=4, rhs_lanes=1, rhs_per_lane=4, block=256, window=64),
+               dict(engine=S.ENGINE_LANE, block=256, window=512)]
+
+
+def _rhs_id(c):
+    if c is None:
+        return "default"
+    if c.get("engine") == S.ENGINE_LANE:
+        return "lane-B{block}-W{window}".format(**c)
+    return "G{lanes_per_column}-RL{rhs_lanes}-r{rhs_per_lane}".format(**c)
 
 
 @pytest.mark.parametrize("rhs", [2, 8, 33, 130])
-@pytest.mark.parametrize("cfg", RHS_CONFIGS, ids=lambda c: "default" if c is None else
-                         "G{lanes_per_column}-RL{rhs_lanes}-r{rhs_per_lane}".format(**c))
+@pytest.mark.parametrize("cfg", RHS_CONFIGS, ids=_rhs_id)
 def test_multi_rhs_matches_oracle(rhs, cfg):
     f = mx.laplacian3d_lower(16, 12, 10)
     X = mx.x_normal(f.n, rhs, seed=1)
@@ -252,6 +279,8 @@ def test_fp32_refinement_reaches_fp64_accuracy(rhs):
     X = mx.x_normal(f.n, rhs, seed=1) if rhs > 1 else mx.x_uniform(f.n, seed=0)
     B = f.spmv(X)
     plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val, flags=S.PLAN_VALIDATE | S.PLAN_MIXED)
+    if rhs == 1:
+        plan.set_config(1, engine=S.ENGINE_LANE, block=512, window=2048)
     db = S.DeviceArray.from_host(B)
     dx = S.DeviceArray(B.shape, np.float64)
     x64 = oracle_x(f, B, rhs=rhs)
@@ -313,16 +342,18 @@ def test_spin_cap_timeout_then_recovery():
     f = mx.bidiagonal(200_000)
     B = f.spmv(mx.x_uniform(f.n, seed=1))
     plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
-    plan.set_config(1, lanes_per_column=1, block=256, window=256)
-    plan.set_spin_cap(2)
     db = S.DeviceArray.from_host(B)
     dx = S.DeviceArray(B.shape, np.float64)
-    plan.solve(db, dx)
-    assert plan.check() == -9  # SPTRSV_ERR_TIMEOUT; the plan state is reset
-    plan.set_spin_cap(0)
-    plan.solve(db, dx)
-    assert plan.check() == 0
-    assert_close(dx.to_host(), oracle_x(f, B), 1e-12)
+    for cfg in (dict(lanes_per_column=1, block=256, window=256),
+                dict(engine=S.ENGINE_LANE, block=256, window=256)):
+        plan.set_config(1, **cfg)
+        plan.set_spin_cap(2)
+        plan.solve(db, dx)
+        assert plan.check() == -9  # SPTRSV_ERR_TIMEOUT; the plan state is reset
+        plan.set_spin_cap(0)
+        plan.solve(db, dx)
+        assert plan.check() == 0
+        assert_close(dx.to_host(), oracle_x(f, B), 1e-12)
 
 
 def test_autotune_picks_a_working_config():
