@@ -27,6 +27,14 @@ template <typename T>
 __device__ __forceinline__ void st_agent(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Publish a value to another workgroup: an agent-scope read-modify-write is
+// performed where all XCDs see it, whereas a plain store may linger in the
+// writer's L2 until that line is written back.  The old value is unused, so
+// the exchange is issued without a return and nothing waits on it.
+template <typename T>
+__device__ __forceinline__ void xchg_agent(T *p, T v) {
+  (void)__hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 template <typename T>
 __device__ __forceinline__ void add_agent(T *p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

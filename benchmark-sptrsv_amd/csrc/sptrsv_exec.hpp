@@ -192,7 +192,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
       ++spins;
       if ((spins & 1023) == 0 && ld_agent(a.err) != 0) active = false;
       if (spins > a.spin_cap) {
-        st_agent(a.err, 1);
+        xchg_agent(a.err, 1);
         active = false;
       }
     }

@@ -14,11 +14,13 @@
 //    store or load would otherwise stall every poll of the wavefront).
 //    Column metadata and the first kPF off-diagonal entries of 64 columns are
 //    loaded into registers once per batch.
-//  * A contribution to a row of a later window is stored into that entry's own
-//    slot in HBM.  The slot holds a NaN sentinel while empty, so the stored
-//    value is its own arrival flag: one relaxed store by the producer, one
-//    relaxed load by the consumer, no atomic read-modify-write, no fence and
-//    no counter.  Slots are grouped by row (rptr) and summed in slot order.
+//  * A contribution to a row of a later window is written into that entry's
+//    own slot in HBM.  The slot holds a NaN sentinel while empty, so the
+//    written value is its own arrival flag: one relaxed exchange by the
+//    producer (an RMW, so it lands where every XCD sees it; its result is
+//    unused, so nothing waits for it), one relaxed load by the consumer, no
+//    fence and no counter.  Slots are grouped by row (rptr) and summed in slot
+//    order.
 //  * Wavefront 0 of each workgroup is the receiver: it polls the slots of the
 //    window's rows, folds complete sums into LDS and resets the slots to the
 //    sentinel, so every solve leaves the slots ready for the next one.
@@ -80,7 +82,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     const int k = base + lane;
     bool active = k < wlen;
     int lo = 0, hi = 0, total = 0;
-    T diag = T(1);
+    T diag = T(1), inv = T(1);
     int tg[kPF];
     T vv[kPF];
     if (active) {
@@ -88,6 +90,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       const int c = BWD ? n - 1 - p : p;
       const int f = a.colPtr[c], l = a.colPtr[c + 1];
       diag = a.val[BWD ? l - 1 : f];
+      inv = T(1) / diag;  // off the critical path
       lo = BWD ? f : f + 1;
       hi = BWD ? l - 1 : l;
       const int2 nd = a.need[c];
@@ -103,7 +106,11 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     while (__ballot(active) != 0) {
       if (active && ld_wg(&s_cnt[k]) == total) {
         fence_acq_wg();
-        const T xv = s_val[k] / diag;
+        // quotient by the precomputed reciprocal plus one residual correction
+        // (two fused multiply-adds instead of a full division on the chain)
+        const T q = s_val[k];
+        T xv = q * inv;
+        xv = __builtin_fma(__builtin_fma(-xv, diag, q), inv, xv);
         s_val[k] = xv;  // the column is final: its slot now holds x
 #pragma unroll
         for (int i = 0; i < kPF; i++)
@@ -119,7 +126,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
             if (tg[i] < 0)
               add_wg(&s_cnt[-tg[i] - 1], 1);
             else
-              st_agent(&slots[tg[i]], xv * vv[i]);
+              xchg_agent(&slots[tg[i]], xv * vv[i]);
           }
         }
         for (int j = lo + kPF; j < hi; j++) {
@@ -127,14 +134,14 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           if (t < 0)
             add_wg(&s_cnt[-t - 1], 1);
           else
-            st_agent(&slots[t], xv * a.val[j]);
+            xchg_agent(&slots[t], xv * a.val[j]);
         }
         active = false;
       } else if (active) {
         ++spins;
         if ((spins & 4095) == 0 && ld_agent(a.err) != 0) active = false;
         if (spins > a.spin_cap) {
-          st_agent(a.err, 1);
+          xchg_agent(a.err, 1);
           active = false;
         }
       }
@@ -190,7 +197,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
         ++spins;
         if ((spins & 1023) == 0 && ld_agent(a.err) != 0) active = false;
         if (spins > a.spin_cap) {
-          st_agent(a.err, 1);
+          xchg_agent(a.err, 1);
           active = false;
         }
       }
