@@ -28,9 +28,12 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Publish a value to another workgroup: an agent-scope read-modify-write is
-// performed where all XCDs see it, whereas a plain store may linger in the
-// writer's L2 until that line is written back.  The old value is unused, so
-// the exchange is issued without a return and nothing waits on it.
+// performed where all XCDs see it.  A plain store instead allocates the line
+// in the writer's XCD L2; sc1 polls issued from that XCD can then keep hitting
+// that copy (stale for every other word of the line) until it is evicted or
+// an acquire fence invalidates it.  Every in-kernel write to a line that other
+// workgroups poll is therefore an RMW.  The old value is unused, so the
+// exchange is issued without a return and nothing waits on it.
 template <typename T>
 __device__ __forceinline__ void xchg_agent(T *p, T v) {
   (void)__hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -128,7 +128,9 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
             if (ls == 0) {
               T *gp = &a.gsum[(tile_rows + c) * R + col_in_tile];
               g = ld_agent(gp);
-              st_agent(gp, T(0));
+              // RMW reset: a plain store would cache the line in this XCD's
+              // L2, where polls of neighbouring rows would then read it
+              xchg_agent(gp, T(0));
             }
             if (NS > 1) g = __shfl(g, leader + lr);
             left = g + left;
@@ -137,7 +139,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
           if (ls == 0 && ridx < a.rhs) a.x[(size_t)c * a.rhs + ridx] = xv[q];
         }
       }
-      if (gl == 0 && nd.y != 0) st_agent(&a.gcnt[tile_rows + c], 0);
+      if (gl == 0 && nd.y != 0) xchg_agent(&a.gcnt[tile_rows + c], 0);
 
       // partial sums: LDS for rows in this window, HBM for later windows
       bool remote = false;
@@ -229,8 +231,8 @@ __global__ void __launch_bounds__(1024) k_exec(ExecArgs<T> a) {
   if (threadIdx.x == 0) {
     const int done = atomicAdd(a.ticket + 1, 1);
     if (done == (int)gridDim.x - 1) {
-      st_agent(a.ticket, 0);
-      st_agent(a.ticket + 1, 0);
+      xchg_agent(a.ticket, 0);
+      xchg_agent(a.ticket + 1, 0);
     }
   }
 }

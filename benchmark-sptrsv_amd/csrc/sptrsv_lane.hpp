@@ -188,7 +188,9 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
         }
       }
       if (full) {
-        for (int s = r0; s < r1; s++) st_agent(&slots[s], slot_empty_value<T>());
+        // reset with an RMW too: a plain store would pull the line into this
+        // XCD's L2, where later polls of neighbouring slots would read it
+        for (int s = r0; s < r1; s++) xchg_agent(&slots[s], slot_empty_value<T>());
         add_wg(&s_val[k], -sum);
         fence_rel_wg();
         add_wg(&s_cnt[k], r1 - r0);
@@ -253,8 +255,8 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   if (threadIdx.x == 0) {
     const int done = atomicAdd(a.ticket + 1, 1);
     if (done == (int)gridDim.x - 1) {
-      st_agent(a.ticket, 0);
-      st_agent(a.ticket + 1, 0);
+      xchg_agent(a.ticket, 0);
+      xchg_agent(a.ticket + 1, 0);
     }
   }
 }
