@@ -38,6 +38,20 @@ template <typename T>
 __device__ __forceinline__ void xchg_agent(T *p, T v) {
   (void)__hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Read the current value of a location other workgroups update, through a
+// read-modify-write that leaves it unchanged.  Unlike a relaxed load (a sc1
+// global load, which an L2 line already resident in this XCD can serve), an
+// RMW is performed at the coherence point and returns the latest value in the
+// location's modification order, so a spin-wait on it cannot stall on a stale
+// cached copy.
+__device__ __forceinline__ int rmw_read(int *p) {
+  return __hip_atomic_fetch_or(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Take the value of a slot and leave `empty` behind, in one RMW.
+template <typename T>
+__device__ __forceinline__ T xchg_take(T *p, T empty) {
+  return __hip_atomic_exchange(p, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 template <typename T>
 __device__ __forceinline__ void add_agent(T *p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -24,8 +24,11 @@
 //    sides.  The 64/G column groups of a wavefront make progress independently
 //    inside one polling loop, so no lane ever spins on a result its own
 //    wavefront has yet to produce.
-//  * Global counters and partial sums are returned to zero by their consumer,
-//    so a solve needs no memset and the next solve starts from a clean state.
+//  * Global counters are polled with an RMW that leaves them unchanged, and
+//    global partial sums are taken with exchange(sum, 0): RMWs are performed
+//    at the coherence point, whereas a plain sc1 load may be served by a line
+//    resident in this XCD's L2.  Counters and sums are returned to zero by
+//    their consumer, so a solve needs no memset and leaves a clean state.
 #pragma once
 
 #include "sptrsv_device.hpp"
@@ -104,7 +107,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
     bool ready = false;
     if (active) {
       ready = ld_wg(&s_cnt[k]) == nd.x;
-      if (ready && nd.y != 0) ready = ld_agent(&a.gcnt[tile_rows + c]) == nd.y;
+      if (ready && nd.y != 0) ready = rmw_read(&a.gcnt[tile_rows + c]) == nd.y;
       if (G > 1) ready = __shfl(ready ? 1 : 0, leader) != 0;
     }
     if (ready) {
@@ -125,13 +128,8 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
             // slice 0 reads and clears the global sum; other slices get it
             // by shuffle, so the clear can never overtake a read
             T g = T(0);
-            if (ls == 0) {
-              T *gp = &a.gsum[(tile_rows + c) * R + col_in_tile];
-              g = ld_agent(gp);
-              // RMW reset: a plain store would cache the line in this XCD's
-              // L2, where polls of neighbouring rows would then read it
-              xchg_agent(gp, T(0));
-            }
+            if (ls == 0)  // read and clear in one RMW at the coherence point
+              g = xchg_take(&a.gsum[(tile_rows + c) * R + col_in_tile], T(0));
             if (NS > 1) g = __shfl(g, leader + lr);
             left = g + left;
           }
@@ -192,7 +190,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
       spins = 0;
     } else if (active) {
       ++spins;
-      if ((spins & 1023) == 0 && ld_agent(a.err) != 0) active = false;
+      if ((spins & 1023) == 0 && rmw_read(a.err) != 0) active = false;
       if (spins > a.spin_cap) {
         xchg_agent(a.err, 1);
         active = false;
@@ -212,7 +210,7 @@ __global__ void __launch_bounds__(1024) k_exec(ExecArgs<T> a) {
 
   for (;;) {
     if (threadIdx.x == 0) {
-      s_ticket = ld_agent(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
+      s_ticket = rmw_read(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
     }
     __syncthreads();
     const int t = s_ticket;

@@ -21,9 +21,11 @@
 //    unused, so nothing waits for it), one relaxed load by the consumer, no
 //    fence and no counter.  Slots are grouped by row (rptr) and summed in slot
 //    order.
-//  * Wavefront 0 of each workgroup is the receiver: it polls the slots of the
-//    window's rows, folds complete sums into LDS and resets the slots to the
-//    sentinel, so every solve leaves the slots ready for the next one.
+//  * Wavefront 0 of each workgroup is the receiver: it takes the slots of the
+//    window's rows with exchange(slot, EMPTY) -- an RMW, so the poll reads the
+//    coherence point rather than a possibly stale line of this XCD's L2, and
+//    the slot is reset in the same operation -- then folds complete sums into
+//    LDS.  Every solve leaves the slots empty for the next one.
 #pragma once
 
 #include "sptrsv_device.hpp"
@@ -139,7 +141,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         active = false;
       } else if (active) {
         ++spins;
-        if ((spins & 4095) == 0 && ld_agent(a.err) != 0) active = false;
+        if ((spins & 4095) == 0 && rmw_read(a.err) != 0) active = false;
         if (spins > a.spin_cap) {
           xchg_agent(a.err, 1);
           active = false;
@@ -149,6 +151,12 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   }
 }
 
+constexpr int kTake = 8;  // slots of one row taken per poll
+
+// Receiver: lane l walks the window's columns l, l + 64, ...  A column's
+// remote slots are taken in groups of kTake with exchange(slot, EMPTY), which
+// both polls and resets a slot; a group is summed in slot order once all of
+// it has arrived, so the remote sum has a fixed order.
 template <typename T, bool BWD>
 __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int wlo, int wlen,
@@ -156,48 +164,54 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   const int lane = __lane_id();
   const int n = a.n;
   T *slots = a.slots + (size_t)tile * a.nslots;
+  const T empty = slot_empty_value<T>();
   int k = lane;
-  int r0 = 0, r1 = 0;
+  int s = 0, r1 = 0, m = 0;
+  unsigned got = 0;
+  T part[kTake];
+  T sum = T(0);
   auto load = [&]() {
     const int p = wlo + k;
     const int c = BWD ? n - 1 - p : p;
-    r0 = a.rptr[c];
+    s = a.rptr[c];
     r1 = a.rptr[c + 1];
+    m = r1 - s;
+    got = 0;
+    sum = T(0);
   };
   bool active = k < wlen;
   if (active) load();
   long long spins = 0;
   while (__ballot(active) != 0) {
-    bool done = r0 == r1;
+    bool done = active && s == r1;
     if (active && !done) {
-      T sum = T(0);
-      bool full = true;
-      for (int s = r0; s < r1 && full; s += 4) {
-        T v[4];
+      const int g = min(kTake, r1 - s);
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          v[q] = s + q < r1 ? ld_agent(&slots[s + q]) : T(0);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          if (s + q < r1) {
-            if (slot_empty(v[q]))
-              full = false;
-            else
-              sum += v[q];
+      for (int q = 0; q < kTake; q++) {
+        if (q < g && !(got & (1u << q))) {
+          const T v = xchg_take(&slots[s + q], empty);
+          if (!slot_empty(v)) {
+            part[q] = v;
+            got |= 1u << q;
           }
         }
       }
-      if (full) {
-        // reset with an RMW too: a plain store would pull the line into this
-        // XCD's L2, where later polls of neighbouring slots would read it
-        for (int s = r0; s < r1; s++) xchg_agent(&slots[s], slot_empty_value<T>());
-        add_wg(&s_val[k], -sum);
-        fence_rel_wg();
-        add_wg(&s_cnt[k], r1 - r0);
-        done = true;
+      if (got == (1u << g) - 1) {
+#pragma unroll
+        for (int q = 0; q < kTake; q++)
+          if (q < g) sum += part[q];
+        s += g;
+        got = 0;
+        spins = 0;
+        if (s == r1) {
+          add_wg(&s_val[k], -sum);
+          fence_rel_wg();
+          add_wg(&s_cnt[k], m);
+          done = true;
+        }
       } else {
         ++spins;
-        if ((spins & 1023) == 0 && ld_agent(a.err) != 0) active = false;
+        if ((spins & 1023) == 0 && rmw_read(a.err) != 0) active = false;
         if (spins > a.spin_cap) {
           xchg_agent(a.err, 1);
           active = false;
@@ -226,7 +240,7 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
 
   for (;;) {
     if (threadIdx.x == 0)
-      s_ticket = ld_agent(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
+      s_ticket = rmw_read(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
     __syncthreads();
     const int t = s_ticket;
     if (t >= total) break;
