@@ -78,6 +78,27 @@ __device__ __forceinline__ void fence_rel_wg() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 }
 
+// Exponential back-off for a wavefront whose spin-wait made no progress:
+// sleeps 64 * level cycles, level doubling up to kBackoffMax, and resets when
+// the wavefront makes progress.  Keeps thousands of waiting wavefronts from
+// flooding the coherence point with polls.
+constexpr int kBackoffMax = 16;
+__device__ __forceinline__ void backoff(int &level, bool progressed) {
+  if (progressed) {
+    level = 0;
+    return;
+  }
+  switch (level) {  // s_sleep takes an immediate
+    case 0: break;
+    case 1: __builtin_amdgcn_s_sleep(1); break;
+    case 2: __builtin_amdgcn_s_sleep(2); break;
+    case 4: __builtin_amdgcn_s_sleep(4); break;
+    case 8: __builtin_amdgcn_s_sleep(8); break;
+    default: __builtin_amdgcn_s_sleep(16); break;
+  }
+  level = level == 0 ? 1 : min(level * 2, kBackoffMax);
+}
+
 // Largest index i in [0, len) with a[i] <= key (a ascending, a[0] <= key).
 __device__ __forceinline__ int upper_bound_minus1(const int *a, int len,
                                                   int key) {

@@ -102,6 +102,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
   if (active) load_column(k);
 
   long long spins = 0;
+  int level = 0;
   for (;;) {
     if (__ballot(active) == 0) break;
     bool ready = false;
@@ -190,12 +191,15 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
       spins = 0;
     } else if (active) {
       ++spins;
-      if ((spins & 1023) == 0 && rmw_read(a.err) != 0) active = false;
+      if ((spins & 255) == 0 && rmw_read(a.err) != 0) active = false;
       if (spins > a.spin_cap) {
         xchg_agent(a.err, 1);
         active = false;
       }
     }
+    // back off only while this wavefront waits on HBM counters
+    const bool waits_global = __ballot(active && !ready && nd.y != 0) != 0;
+    backoff(level, __ballot(ready) != 0 || !waits_global);
   }
 }
 

@@ -151,12 +151,14 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   }
 }
 
-constexpr int kTake = 8;  // slots of one row taken per poll
+constexpr int kTake = 8;  // slots of one row taken per group
 
 // Receiver: lane l walks the window's columns l, l + 64, ...  A column's
-// remote slots are taken in groups of kTake with exchange(slot, EMPTY), which
-// both polls and resets a slot; a group is summed in slot order once all of
-// it has arrived, so the remote sum has a fixed order.
+// remote slots are taken in groups of up to kTake with exchange(slot, EMPTY),
+// which both polls and resets a slot.  Only the first slot of a group is
+// polled until it arrives; the rest of the group is taken after it, and the
+// group is summed in slot order once complete, so the remote sum has a fixed
+// order.  A receiver wavefront that took nothing in an iteration backs off.
 template <typename T, bool BWD>
 __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int wlo, int wlen,
@@ -182,13 +184,17 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   bool active = k < wlen;
   if (active) load();
   long long spins = 0;
+  int level = 0;
   while (__ballot(active) != 0) {
     bool done = active && s == r1;
+    bool progressed = false;
     if (active && !done) {
       const int g = min(kTake, r1 - s);
+      const unsigned before = got;
 #pragma unroll
       for (int q = 0; q < kTake; q++) {
-        if (q < g && !(got & (1u << q))) {
+        // slot 0 of the group first; the others once slot 0 is in
+        if (q < g && !(got & (1u << q)) && (q == 0 || (got & 1u))) {
           const T v = xchg_take(&slots[s + q], empty);
           if (!slot_empty(v)) {
             part[q] = v;
@@ -196,6 +202,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
           }
         }
       }
+      progressed = got != before;
       if (got == (1u << g) - 1) {
 #pragma unroll
         for (int q = 0; q < kTake; q++)
@@ -209,9 +216,9 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
           add_wg(&s_cnt[k], m);
           done = true;
         }
-      } else {
+      } else if (!progressed) {
         ++spins;
-        if ((spins & 1023) == 0 && rmw_read(a.err) != 0) active = false;
+        if ((spins & 255) == 0 && rmw_read(a.err) != 0) active = false;
         if (spins > a.spin_cap) {
           xchg_agent(a.err, 1);
           active = false;
@@ -219,11 +226,13 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
       }
     }
     if (done && active) {
+      progressed = true;
       spins = 0;
       k += kWave;
       active = k < wlen;
       if (active) load();
     }
+    backoff(level, __ballot(progressed) != 0);
   }
 }
 
