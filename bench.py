@@ -100,7 +100,22 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB "
+                         "(the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
+
+    if args.calibrate_copy_mib:
+        import sptrsv_amd as S
+        nbytes = args.calibrate_copy_mib << 20
+        src = S.DeviceArray((nbytes,), np.uint8)
+        dst = S.DeviceArray((nbytes,), np.uint8)
+        src.zero()
+        for _ in range(3):
+            S.check(S.lib().sptrsv_memcpy_dtod(dst.ptr, src.ptr, nbytes), "dtod")
+        S.lib().sptrsv_synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path "

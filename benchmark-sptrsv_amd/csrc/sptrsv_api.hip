@@ -24,9 +24,9 @@ namespace {
 
 int g_verbose = 0;
 int g_bench_repeat = 100;  // BENCH_REPEAT, SpTRSV_cuda/common.h:13-14
-// polls of one spin-wait before a solve gives up: seconds of polling, far
+// longest single spin-wait before a solve gives up, in microseconds: far
 // beyond any legitimate wait of a valid factor
-constexpr long long kDefaultSpinCap = 1ll << 24;
+constexpr long long kDefaultWaitCapUs = 30ll * 1000 * 1000;
 
 #define HIP_TRY(expr)                                                     \
   do {                                                                    \
@@ -386,7 +386,8 @@ struct sptrsv_plan_s {
   std::map<long long, sptrsv_config_t> configs;  // key: rhs*2 + is_f32_solve
   // mixed precision
   DevBuf val32, csrPtr, csrIdx, csrVal, xtmp, r32, d32;
-  long long spin_cap = kDefaultSpinCap;
+  long long wait_cap_us = kDefaultWaitCapUs;
+  long long clock_khz = 100000;  // wall_clock64 rate (queried at creation)
   int cus = 0;
   size_t lds_max = 65536;
   double analysis_ms = 0.0;
@@ -400,11 +401,14 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   sptrsv_config_t c;
   memset(&c, 0, sizeof(c));
   if (rhs <= 1) {
-    c.lanes_per_column = 64;  // one wavefront per column, as in the reference
+    // lane engine: one lane per column, 4096-column LDS windows
+    c.engine = SPTRSV_ENGINE_LANE;
+    c.lanes_per_column = 1;
     c.rhs_lanes = 1;
     c.rhs_per_lane = 1;
-    c.block = 1024;
-    c.window = 16;
+    c.block = 256;
+    c.window = 4096;
+    if ((size_t)c.window * 12 > p->lds_max) c.window = 2048;
   } else {
     int rl = 1;
     while (rl < rhs && rl < 64) rl <<= 1;
@@ -416,7 +420,6 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   }
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
-  (void)p;
   return c;
 }
 
@@ -741,7 +744,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.nslots = L.ws->nslots;
     la.ticket = p->sync.as<int>();
     la.err = p->sync.as<int>() + 2;
-    la.spin_cap = p->spin_cap;
+    la.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
     LaneKernel<T> kern = p->subst == SPTRSV_SUBSTITUTION_BACKWARD
                              ? k_exec_lane<T, true>
                              : k_exec_lane<T, false>;
@@ -767,7 +770,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
   a.rpl = c.rhs_per_lane;
   a.ticket = p->sync.as<int>();
   a.err = p->sync.as<int>() + 2;
-  a.spin_cap = p->spin_cap;
+  a.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
   const int maxrpl = c.rhs_per_lane == 1 ? 1 : 4;
   ExecKernel<T> kern = pick_exec_dir<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
                                         c.lanes_per_column, c.rhs_lanes, maxrpl);
@@ -855,6 +858,14 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
   p->val = val;
   p->avg_offdiag = n > 0 ? (double)(nnz - n) / n : 0.0;
   int st = device_props(&p->cus, &p->lds_max);
+  if (st == SPTRSV_OK) {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) ==
+            hipSuccess &&
+        khz > 0)
+      p->clock_khz = khz;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (st == SPTRSV_OK && (hipEventCreate(&e0) != hipSuccess ||
                           hipEventCreate(&e1) != hipSuccess))
@@ -990,10 +1001,13 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
   return out;
 }
 
+// Mean solve time of configuration c.  A candidate whose warm-up solve takes
+// more than twice `give_up_ms` is not timed further (its warm-up time is
+// returned), so pathological configurations cost one solve, not six.
 template <typename T>
 int time_config(sptrsv_plan_s *p, const sptrsv_config_t &c, int rhs,
                 bool f32solve, const T *val, const T *b, T *x, float *ms,
-                hipStream_t s) {
+                hipStream_t s, float give_up_ms) {
   const long long key = (long long)rhs * 2 + (f32solve ? 1 : 0);
   auto saved = p->configs.find(key);
   bool had = saved != p->configs.end();
@@ -1005,10 +1019,18 @@ int time_config(sptrsv_plan_s *p, const sptrsv_config_t &c, int rhs,
   if (st == SPTRSV_OK && (hipEventCreate(&e0) != hipSuccess ||
                           hipEventCreate(&e1) != hipSuccess))
     st = SPTRSV_ERR_HIP;
-  if (st == SPTRSV_OK) st = launch_exec<T>(p, L, val, b, x, rhs, s);  // warm-up
-  if (st == SPTRSV_OK) st = sptrsv_plan_check(p, s);
-  const int reps = 5;
+  float warm = 0.f;
   if (st == SPTRSV_OK) {
+    hipEventRecord(e0, s);
+    st = launch_exec<T>(p, L, val, b, x, rhs, s);  // warm-up
+    hipEventRecord(e1, s);
+  }
+  if (st == SPTRSV_OK) st = sptrsv_plan_check(p, s);
+  if (st == SPTRSV_OK && hipEventElapsedTime(&warm, e0, e1) == hipSuccess &&
+      warm > 2.f * give_up_ms) {
+    *ms = warm;
+  } else if (st == SPTRSV_OK) {
+    const int reps = 5;
     hipEventRecord(e0, s);
     for (int r = 0; r < reps && st == SPTRSV_OK; r++)
       st = launch_exec<T>(p, L, val, b, x, rhs, s);
@@ -1376,7 +1398,7 @@ int sptrsv_plan_reset(sptrsv_plan_t p, void *stream) {
 
 int sptrsv_plan_set_spin_cap(sptrsv_plan_t p, long long cap) {
   if (!p || cap < 0) return SPTRSV_ERR_INVALID_ARG;
-  p->spin_cap = cap == 0 ? kDefaultSpinCap : cap;
+  p->wait_cap_us = cap == 0 ? kDefaultWaitCapUs : cap;
   return SPTRSV_OK;
 }
 
@@ -1443,9 +1465,10 @@ int sptrsv_plan_autotune(sptrsv_plan_t p, int rhs, int mode,
   for (const sptrsv_config_t &c : cands) {
     float ms = 0.f;
     int r = f32 ? time_config<float>(p, c, rhs, true, (const float *)val,
-                                     bb.as<float>(), xx.as<float>(), &ms, s)
+                                     bb.as<float>(), xx.as<float>(), &ms, s, bms)
                 : time_config<double>(p, c, rhs, false, (const double *)val,
-                                      bb.as<double>(), xx.as<double>(), &ms, s);
+                                      bb.as<double>(), xx.as<double>(), &ms, s,
+                                      bms);
     if (r == SPTRSV_OK && ms > 0.f && ms < bms) {
       bms = ms;
       bc = c;

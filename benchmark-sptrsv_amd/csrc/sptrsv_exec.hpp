@@ -49,7 +49,7 @@ struct ExecArgs {
   int n, rhs, nwin, ntiles, maxw, rpl;
   int *ticket;                    // [2]: next ticket, finished workgroups
   int *err;                       // set to 1 on a spin-wait timeout
-  long long spin_cap;
+  long long wait_cap;              // longest single wait, wall-clock ticks
 };
 
 constexpr int kPrefetch = 4;  // off-diagonal entries per lane loaded early
@@ -102,6 +102,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
   if (active) load_column(k);
 
   long long spins = 0;
+  unsigned long long t0 = wall_clock64();
   int level = 0;
   for (;;) {
     if (__ballot(active) == 0) break;
@@ -189,10 +190,11 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
       active = k < wlen;
       if (active) load_column(k);
       spins = 0;
-    } else if (active) {
-      ++spins;
-      if ((spins & 255) == 0 && rmw_read(a.err) != 0) active = false;
-      if (spins > a.spin_cap) {
+      t0 = wall_clock64();
+    } else if (active && (++spins & 255) == 0) {
+      if (rmw_read(a.err) != 0) {
+        active = false;
+      } else if ((long long)(wall_clock64() - t0) > a.wait_cap) {
         xchg_agent(a.err, 1);
         active = false;
       }

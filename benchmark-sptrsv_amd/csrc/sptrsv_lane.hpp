@@ -68,7 +68,7 @@ struct LaneArgs {
   int n, rhs, nwin, ntiles, maxw, nslots;
   int *ticket;
   int *err;
-  long long spin_cap;
+  long long wait_cap;  // longest single wait, wall-clock ticks
 };
 
 constexpr int kPF = 4;  // off-diagonal entries per column held in registers
@@ -105,6 +105,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       }
     }
     long long spins = 0;
+    const unsigned long long t0 = wall_clock64();
     while (__ballot(active) != 0) {
       if (active && ld_wg(&s_cnt[k]) == total) {
         fence_acq_wg();
@@ -139,10 +140,10 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
             xchg_agent(&slots[t], xv * a.val[j]);
         }
         active = false;
-      } else if (active) {
-        ++spins;
-        if ((spins & 4095) == 0 && rmw_read(a.err) != 0) active = false;
-        if (spins > a.spin_cap) {
+      } else if (active && (++spins & 4095) == 0) {
+        if (rmw_read(a.err) != 0) {
+          active = false;
+        } else if ((long long)(wall_clock64() - t0) > a.wait_cap) {
           xchg_agent(a.err, 1);
           active = false;
         }
@@ -184,6 +185,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   bool active = k < wlen;
   if (active) load();
   long long spins = 0;
+  unsigned long long t0 = wall_clock64();
   int level = 0;
   while (__ballot(active) != 0) {
     bool done = active && s == r1;
@@ -210,16 +212,17 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
         s += g;
         got = 0;
         spins = 0;
+        t0 = wall_clock64();
         if (s == r1) {
           add_wg(&s_val[k], -sum);
           fence_rel_wg();
           add_wg(&s_cnt[k], m);
           done = true;
         }
-      } else if (!progressed) {
-        ++spins;
-        if ((spins & 255) == 0 && rmw_read(a.err) != 0) active = false;
-        if (spins > a.spin_cap) {
+      } else if (!progressed && (++spins & 63) == 0) {
+        if (rmw_read(a.err) != 0) {
+          active = false;
+        } else if ((long long)(wall_clock64() - t0) > a.wait_cap) {
           xchg_agent(a.err, 1);
           active = false;
         }
@@ -228,6 +231,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
     if (done && active) {
       progressed = true;
       spins = 0;
+      t0 = wall_clock64();
       k += kWave;
       active = k < wlen;
       if (active) load();

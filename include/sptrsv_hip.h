@@ -260,8 +260,9 @@ int sptrsv_graph_destroy(sptrsv_graph_t graph);
 int sptrsv_plan_check(sptrsv_plan_t plan, void *stream);
 /* Clear the plan's scratch state (counters, partial sums, tickets). */
 int sptrsv_plan_reset(sptrsv_plan_t plan, void *stream);
-/* Cap on polls of one spin-wait before a solve gives up (0 = default). */
-int sptrsv_plan_set_spin_cap(sptrsv_plan_t plan, long long cap);
+/* Longest one spin-wait may last, in microseconds of wall clock, before the
+ * solve gives up and reports SPTRSV_ERR_TIMEOUT (0 = default, 30 s). */
+int sptrsv_plan_set_spin_cap(sptrsv_plan_t plan, long long cap_us);
 
 typedef struct {
   int n, nnz, substitution, value_type, nsys;

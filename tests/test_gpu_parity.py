@@ -347,7 +347,7 @@ def test_spin_cap_timeout_then_recovery():
     for cfg in (dict(lanes_per_column=1, block=256, window=256),
                 dict(engine=S.ENGINE_LANE, block=256, window=256)):
         plan.set_config(1, **cfg)
-        plan.set_spin_cap(2)
+        plan.set_spin_cap(1)  # 1 us: the chain's later columns wait longer
         plan.solve(db, dx)
         assert plan.check() == -9  # SPTRSV_ERR_TIMEOUT; the plan state is reset
         plan.set_spin_cap(0)
