@@ -76,7 +76,12 @@ struct DevBuf {
     }
     HIP_TRY(hipMalloc(&p, want));
     bytes = want;
-    if (zero) HIP_TRY(hipMemset(p, 0, want));
+    if (zero) {
+      // hipMemset runs on the legacy NULL stream, which does not order with
+      // non-blocking streams: finish it before any caller's stream uses p
+      HIP_TRY(hipMemset(p, 0, want));
+      HIP_TRY(hipDeviceSynchronize());
+    }
     return SPTRSV_OK;
   }
   void release() {
@@ -1119,11 +1124,17 @@ int sptrsv_memcpy_dtoh(void *dst, const void *src, size_t b) {
   return SPTRSV_OK;
 }
 int sptrsv_memcpy_dtod(void *dst, const void *src, size_t b) {
-  if (b) HIP_TRY(hipMemcpy(dst, src, b, hipMemcpyDeviceToDevice));
+  if (b) {
+    HIP_TRY(hipMemcpy(dst, src, b, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipDeviceSynchronize());  // complete before any stream uses dst
+  }
   return SPTRSV_OK;
 }
 int sptrsv_memset(void *d, int v, size_t b) {
-  if (b) HIP_TRY(hipMemset(d, v, b));
+  if (b) {
+    HIP_TRY(hipMemset(d, v, b));
+    HIP_TRY(hipDeviceSynchronize());  // complete before any stream uses d
+  }
   return SPTRSV_OK;
 }
 int sptrsv_synchronize(void) { HIP_TRY(hipDeviceSynchronize()); return SPTRSV_OK; }

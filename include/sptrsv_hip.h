@@ -104,7 +104,8 @@ void sptrsv_set_bench_repeat(int repeat); /* >= 1; BENCH_REPEAT, common.h:13-14 
 
 /* ========================================================================
  * Device-memory helpers (thin wrappers over the HIP runtime the library uses,
- * so that C, ctypes, cgo, ... callers need no HIP headers).
+ * so that C, ctypes, cgo, ... callers need no HIP headers).  Copies and
+ * memsets are complete when they return, so any stream may use the data.
  * ======================================================================== */
 int sptrsv_device_count(int *count);
 int sptrsv_set_device(int device);
