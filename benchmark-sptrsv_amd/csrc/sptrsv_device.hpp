@@ -99,6 +99,38 @@ __device__ __forceinline__ void backoff(int &level, bool progressed) {
   level = level == 0 ? 1 : min(level * 2, kBackoffMax);
 }
 
+// Per-wavefront stall watch for spin-waits.  Only lane 0 touches the clock
+// and the shared error word, and only every 1024 stalled iterations, so the
+// watch adds no traffic to the coherence point (one shared word polled by
+// every lane of every wavefront would serialise a memory channel).
+// step() returns true when the wavefront must stop: another wavefront raised
+// the error, or this one has made no progress for longer than cap ticks.
+struct StallWatch {
+  unsigned long long t_progress;
+  unsigned stalled;
+  __device__ explicit StallWatch() : t_progress(wall_clock64()), stalled(0) {}
+  __device__ __forceinline__ bool step(bool progressed, int *err,
+                                       long long cap) {
+    if (progressed) {
+      stalled = 0;
+      t_progress = wall_clock64();
+      return false;
+    }
+    if ((++stalled & 1023) != 0) return false;
+    int stop = 0;
+    if (__lane_id() == 0) {
+      const unsigned long long now = wall_clock64();
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        stop = 1;
+      else if ((long long)(now - t_progress) > cap) {
+        __hip_atomic_exchange(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stop = 1;
+      }
+    }
+    return __shfl(stop, 0) != 0;
+  }
+};
+
 // Largest index i in [0, len) with a[i] <= key (a ascending, a[0] <= key).
 __device__ __forceinline__ int upper_bound_minus1(const int *a, int len,
                                                   int key) {

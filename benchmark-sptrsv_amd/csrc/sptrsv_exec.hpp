@@ -101,8 +101,7 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
   };
   if (active) load_column(k);
 
-  long long spins = 0;
-  unsigned long long t0 = wall_clock64();
+  StallWatch watch;
   int level = 0;
   for (;;) {
     if (__ballot(active) == 0) break;
@@ -189,19 +188,12 @@ __device__ __forceinline__ void exec_window(const ExecArgs<T> &a, T *s_sum,
       k += ng;
       active = k < wlen;
       if (active) load_column(k);
-      spins = 0;
-      t0 = wall_clock64();
-    } else if (active && (++spins & 255) == 0) {
-      if (rmw_read(a.err) != 0) {
-        active = false;
-      } else if ((long long)(wall_clock64() - t0) > a.wait_cap) {
-        xchg_agent(a.err, 1);
-        active = false;
-      }
     }
     // back off only while this wavefront waits on HBM counters
+    const bool any = __ballot(ready) != 0;
     const bool waits_global = __ballot(active && !ready && nd.y != 0) != 0;
-    backoff(level, __ballot(ready) != 0 || !waits_global);
+    backoff(level, any || !waits_global);
+    if (watch.step(any, a.err, a.wait_cap)) active = false;
   }
 }
 
@@ -216,7 +208,7 @@ __global__ void __launch_bounds__(1024) k_exec(ExecArgs<T> a) {
 
   for (;;) {
     if (threadIdx.x == 0) {
-      s_ticket = rmw_read(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
+      s_ticket = ld_agent(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
     }
     __syncthreads();
     const int t = s_ticket;

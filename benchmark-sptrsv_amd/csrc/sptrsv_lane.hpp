@@ -104,9 +104,9 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         vv[i] = j < hi ? a.val[j] : T(0);
       }
     }
-    long long spins = 0;
-    const unsigned long long t0 = wall_clock64();
+    StallWatch watch;
     while (__ballot(active) != 0) {
+      bool progressed = false;
       if (active && ld_wg(&s_cnt[k]) == total) {
         fence_acq_wg();
         // quotient by the precomputed reciprocal plus one residual correction
@@ -140,14 +140,9 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
             xchg_agent(&slots[t], xv * a.val[j]);
         }
         active = false;
-      } else if (active && (++spins & 4095) == 0) {
-        if (rmw_read(a.err) != 0) {
-          active = false;
-        } else if ((long long)(wall_clock64() - t0) > a.wait_cap) {
-          xchg_agent(a.err, 1);
-          active = false;
-        }
+        progressed = true;
       }
+      if (watch.step(__ballot(progressed) != 0, a.err, a.wait_cap)) active = false;
     }
   }
 }
@@ -184,8 +179,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   };
   bool active = k < wlen;
   if (active) load();
-  long long spins = 0;
-  unsigned long long t0 = wall_clock64();
+  StallWatch watch;
   int level = 0;
   while (__ballot(active) != 0) {
     bool done = active && s == r1;
@@ -211,32 +205,23 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
           if (q < g) sum += part[q];
         s += g;
         got = 0;
-        spins = 0;
-        t0 = wall_clock64();
         if (s == r1) {
           add_wg(&s_val[k], -sum);
           fence_rel_wg();
           add_wg(&s_cnt[k], m);
           done = true;
         }
-      } else if (!progressed && (++spins & 63) == 0) {
-        if (rmw_read(a.err) != 0) {
-          active = false;
-        } else if ((long long)(wall_clock64() - t0) > a.wait_cap) {
-          xchg_agent(a.err, 1);
-          active = false;
-        }
       }
     }
     if (done && active) {
       progressed = true;
-      spins = 0;
-      t0 = wall_clock64();
       k += kWave;
       active = k < wlen;
       if (active) load();
     }
-    backoff(level, __ballot(progressed) != 0);
+    const bool any = __ballot(progressed) != 0;
+    backoff(level, any);
+    if (watch.step(any, a.err, a.wait_cap)) active = false;
   }
 }
 
@@ -253,7 +238,7 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
 
   for (;;) {
     if (threadIdx.x == 0)
-      s_ticket = rmw_read(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
+      s_ticket = ld_agent(a.err) != 0 ? total : atomicAdd(a.ticket, 1);
     __syncthreads();
     const int t = s_ticket;
     if (t >= total) break;
