@@ -375,6 +375,8 @@ struct WindowSet {
   // lane engine: slot ranges per row and slot targets per entry
   DevBuf rptr;    // int[n+1]
   DevBuf etgt;    // int[nnz]
+  DevBuf rlist;   // int4[#columns with remote inputs]
+  DevBuf roff;    // int[nwin+1]
   int nslots = -1;
 };
 
@@ -634,6 +636,41 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     cursor.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
   }
+  // receive lists: the columns with remote inputs, grouped by window
+  {
+    DevBuf pos;
+    ST_TRY(pos.ensure(sizeof(int) * ((size_t)n + 1), false));
+    if (n > 0) {
+      k_remote_flags<<<grid_for(n, 256), 256, 0, s>>>(
+          ws->need.as<int2>(), n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+          pos.as<int>());
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemsetAsync(pos.as<int>() + n, 0, sizeof(int), s));
+    int st = scan_rec(pos.as<int>(), n + 1, s);
+    int nrecv_cols = 0;
+    if (st == SPTRSV_OK &&
+        (hipMemcpyAsync(&nrecv_cols, pos.as<int>() + n, sizeof(int),
+                        hipMemcpyDeviceToHost, s) != hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess))
+      st = SPTRSV_ERR_HIP;
+    if (st == SPTRSV_OK)
+      st = ws->rlist.ensure(sizeof(int4) * (size_t)std::max(nrecv_cols, 1), false);
+    if (st == SPTRSV_OK)
+      st = ws->roff.ensure(sizeof(int) * ((size_t)ws->nwin + 1), false);
+    if (st == SPTRSV_OK) {
+      k_receive_list<<<grid_for(std::max(n, ws->nwin + 1), 256), 256, 0, s>>>(
+          ws->need.as<int2>(), ws->rptr.as<int>(), pos.as<int>(),
+          ws->winPtr.as<int>(), ws->nwin, n,
+          p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+          c.partition == SPTRSV_COL_PARTITION ? c.window : 0,
+          ws->rlist.as<int4>(), ws->roff.as<int>());
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        st = SPTRSV_ERR_HIP;
+    }
+    pos.release();
+    if (st != SPTRSV_OK) return st;
+  }
   ws->nslots = nslots;
   return SPTRSV_OK;
 }
@@ -647,6 +684,13 @@ DevBuf &slot_buf<float>(sptrsv_plan_s *p) { return p->slots32; }
 
 template <typename T>
 using LaneKernel = void (*)(LaneArgs<T>);
+
+// Receiver wavefronts of a lane-engine workgroup: a quarter of them, at least
+// one, always leaving at least one compute wavefront.
+int lane_receivers(int block) {
+  const int waves = block / 64;
+  return std::max(1, std::min(waves / 4, waves - 1));
+}
 
 // Everything a solve launch needs, prepared outside any graph capture.
 struct Launch {
@@ -738,6 +782,9 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.need = L.ws->need.as<int2>();
     la.rptr = L.ws->rptr.as<int>();
     la.winPtr = L.ws->winPtr.as<int>();
+    la.rlist = L.ws->rlist.as<int4>();
+    la.roff = L.ws->roff.as<int>();
+    la.nrecv = lane_receivers(c.block);
     la.slots = slot_buf<T>(p).template as<T>();
     la.b = b;
     la.x = x;
@@ -1315,6 +1362,8 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     kv.second->need.release();
     kv.second->rptr.release();
     kv.second->etgt.release();
+    kv.second->rlist.release();
+    kv.second->roff.release();
     delete kv.second;
   }
   p->windows.clear();
@@ -1429,7 +1478,8 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
   for (const DevBuf *b : bufs) bytes += (long long)b->bytes;
   for (auto &kv : p->windows)
     bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes +
-                         kv.second->rptr.bytes + kv.second->etgt.bytes);
+                         kv.second->rptr.bytes + kv.second->etgt.bytes +
+                         kv.second->rlist.bytes + kv.second->roff.bytes);
   info->scratch_bytes = bytes;
   info->analysis_ms = p->analysis_ms;
   return SPTRSV_OK;

@@ -62,10 +62,14 @@ struct LaneArgs {
   const int2 *__restrict__ need;  // [n] by row: {local, remote} arrivals
   const int *__restrict__ rptr;   // [n+1] slot range of each row
   const int *__restrict__ winPtr; // [nwin + 1] window bounds, solve order
+  const int4 *__restrict__ rlist; // columns with remote inputs, window order:
+                                  // {local index, first slot, slot count, -}
+  const int *__restrict__ roff;   // [nwin + 1] each window's rlist range
   T *slots;                       // [ntiles][nslots]
   const T *__restrict__ b;        // [n][rhs]
   T *__restrict__ x;              // [n][rhs]
   int n, rhs, nwin, ntiles, maxw, nslots;
+  int nrecv;                      // receiver wavefronts per workgroup
   int *ticket;
   int *err;
   long long wait_cap;  // longest single wait, wall-clock ticks
@@ -149,42 +153,46 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 
 constexpr int kTake = 8;  // slots of one row taken per group
 
-// Receiver: lane l walks the window's columns l, l + 64, ...  A column's
-// remote slots are taken in groups of up to kTake with exchange(slot, EMPTY),
-// which both polls and resets a slot.  Only the first slot of a group is
-// polled until it arrives; the rest of the group is taken after it, and the
-// group is summed in slot order once complete, so the remote sum has a fixed
-// order.  A receiver wavefront that took nothing in an iteration backs off.
-template <typename T, bool BWD>
+// Receivers: the first nrecv wavefronts walk the window's receive list (only
+// the columns that have inputs from earlier windows, precomputed per window
+// partition), lane l of receiver r taking entries r*64 + l, stride nrecv*64.
+// A column's slots are taken in groups of up to kTake with
+// exchange(slot, EMPTY), which both polls and resets a slot.  Only the first
+// slot of a group is polled until it arrives; the rest of the group is taken
+// after it, and the group is summed in slot order once complete, so the
+// remote sum has a fixed order.  A receiver wavefront that took nothing in an
+// iteration backs off.
+template <typename T>
 __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
-                                            int *s_cnt, int wlo, int wlen,
-                                            int tile) {
-  const int lane = __lane_id();
-  const int n = a.n;
+                                            int *s_cnt, int win, int tile,
+                                            int rw) {
   T *slots = a.slots + (size_t)tile * a.nslots;
   const T empty = slot_empty_value<T>();
-  int k = lane;
-  int s = 0, r1 = 0, m = 0;
+  const int e_end = a.roff[win + 1];
+  const int stride = a.nrecv * kWave;
+  int e = a.roff[win] + rw * kWave + __lane_id();
+  int k = 0, s = 0, r1 = 0, m = 0;
   unsigned got = 0;
   T part[kTake];
   T sum = T(0);
-  auto load = [&]() {
-    const int p = wlo + k;
-    const int c = BWD ? n - 1 - p : p;
-    s = a.rptr[c];
-    r1 = a.rptr[c + 1];
-    m = r1 - s;
+  int4 next = e < e_end ? a.rlist[e] : make_int4(0, 0, 0, 0);
+  auto take_next = [&]() {
+    k = next.x;
+    s = next.y;
+    m = next.z;
+    r1 = s + m;
     got = 0;
     sum = T(0);
+    if (e + stride < e_end) next = a.rlist[e + stride];  // prefetch
   };
-  bool active = k < wlen;
-  if (active) load();
+  bool active = e < e_end;
+  if (active) take_next();
   StallWatch watch;
   int level = 0;
   while (__ballot(active) != 0) {
-    bool done = active && s == r1;
+    bool done = false;
     bool progressed = false;
-    if (active && !done) {
+    if (active) {
       const int g = min(kTake, r1 - s);
       const unsigned before = got;
 #pragma unroll
@@ -213,11 +221,10 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
         }
       }
     }
-    if (done && active) {
-      progressed = true;
-      k += kWave;
-      active = k < wlen;
-      if (active) load();
+    if (done) {
+      e += stride;
+      active = e < e_end;
+      if (active) take_next();
     }
     const bool any = __ballot(progressed) != 0;
     backoff(level, any);
@@ -233,7 +240,7 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   __shared__ int s_ticket;
   const int total = a.nwin * a.ntiles;
   const int wave = threadIdx.x / kWave;
-  const int ncw = blockDim.x / kWave - 1;
+  const int ncw = blockDim.x / kWave - a.nrecv;
   const int n = a.n;
 
   for (;;) {
@@ -253,10 +260,11 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
       s_cnt[k] = 0;
     }
     __syncthreads();
-    if (wave == 0)
-      lane_receive<T, BWD>(a, s_val, s_cnt, wlo, wlen, tile);
+    if (wave < a.nrecv)
+      lane_receive<T>(a, s_val, s_cnt, win, tile, wave);
     else
-      lane_compute<T, BWD>(a, s_val, s_cnt, wlo, wlen, tile, wave - 1, ncw);
+      lane_compute<T, BWD>(a, s_val, s_cnt, wlo, wlen, tile, wave - a.nrecv,
+                           ncw);
     __syncthreads();
     for (int k = threadIdx.x; k < wlen; k += blockDim.x) {
       const int p = wlo + k;
@@ -303,6 +311,36 @@ __global__ void k_need_remote(const int2 *__restrict__ need, int n, int *out) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += gridDim.x * blockDim.x)
     out[r] = need[r].y;
+}
+
+// flag[p] = 1 when the column at solve position p has remote inputs.
+__global__ void k_remote_flags(const int2 *__restrict__ need, int n,
+                               int backward, int *flag) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += gridDim.x * blockDim.x)
+    flag[p] = need[backward ? n - 1 - p : p].y > 0 ? 1 : 0;
+}
+
+// Receive lists: pos = exclusive scan of the flags (solve order), so each
+// window's entries are contiguous and ascending; roff[w] = pos[winPtr[w]].
+__global__ void k_receive_list(const int2 *__restrict__ need,
+                               const int *__restrict__ rptr,
+                               const int *__restrict__ pos,
+                               const int *__restrict__ winPtr, int nwin, int n,
+                               int backward, int uniform_w, int4 *rlist,
+                               int *roff) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    const int c = backward ? n - 1 - p : p;
+    const int m = need[c].y;
+    if (m > 0) {
+      const int w = uniform_w > 0 ? p / uniform_w
+                                  : upper_bound_minus1(winPtr, nwin + 1, p);
+      rlist[pos[p]] = make_int4(p - winPtr[w], rptr[c], m, 0);
+    }
+  }
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= nwin; w += stride)
+    roff[w] = pos[winPtr[w]];
 }
 
 }  // namespace sptrsv
