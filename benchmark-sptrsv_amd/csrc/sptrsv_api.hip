@@ -372,6 +372,7 @@ struct WindowSet {
   DevBuf need;    // int2[n]
   int nwin = 0;
   int maxw = 0;
+  int uniform_w = 0;  // window width when all windows are equal, else 0
   // lane engine: slot ranges per row and slot targets per entry
   DevBuf rptr;    // int[n+1]
   DevBuf etgt;    // int[nnz]
@@ -520,7 +521,28 @@ int get_windows(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet **out,
   WindowSet *ws = new WindowSet();
   const int n = p->n;
   int st = SPTRSV_OK;
-  if (c.partition == SPTRSV_COL_PARTITION || n == 0) {
+  if (p->nsys > 1 && n > 0) {
+    // batches: every system gets its own windows, so no window mixes two
+    // independent systems and their progress never chains through a window
+    std::vector<int> wp(1, 0);
+    const bool bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
+    for (int q = 0; q < p->nsys; q++) {
+      const int sys = bwd ? p->nsys - 1 - q : q;
+      const int len = p->sys_ptr[sys + 1] - p->sys_ptr[sys];
+      for (int off = 0; off < len; off += c.window)
+        wp.push_back(wp.back() + std::min(c.window, len - off));
+    }
+    ws->nwin = (int)wp.size() - 1;
+    ws->maxw = 0;
+    for (int q = 0; q < ws->nwin; q++)
+      ws->maxw = std::max(ws->maxw, wp[q + 1] - wp[q]);
+    st = ws->winPtr.ensure(sizeof(int) * wp.size(), false);
+    if (st == SPTRSV_OK &&
+        hipMemcpy(ws->winPtr.p, wp.data(), sizeof(int) * wp.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      st = SPTRSV_ERR_HIP;
+  } else if (c.partition == SPTRSV_COL_PARTITION || n == 0) {
+    ws->uniform_w = c.window;
     ws->nwin = n == 0 ? 0 : (n + c.window - 1) / c.window;
     ws->maxw = std::min(c.window, std::max(n, 1));
     st = ws->winPtr.ensure(sizeof(int) * ((size_t)ws->nwin + 1), false);
@@ -579,7 +601,7 @@ int get_windows(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet **out,
       k_need<<<grid_for(n, 256), 256, 0, s>>>(
           p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
           ws->winPtr.as<int>(), ws->nwin,
-          c.partition == SPTRSV_COL_PARTITION ? c.window : 0,
+          ws->uniform_w,
           ws->need.as<int2>());
       if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
     }
@@ -629,7 +651,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     k_lane_targets<<<grid_for(n, 256), 256, 0, s>>>(
         p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
         ws->winPtr.as<int>(), ws->nwin,
-        c.partition == SPTRSV_COL_PARTITION ? c.window : 0, ws->rptr.as<int>(),
+        ws->uniform_w, ws->rptr.as<int>(),
         cursor.as<int>(), ws->etgt.as<int>());
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -663,7 +685,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
           ws->need.as<int2>(), ws->rptr.as<int>(), pos.as<int>(),
           ws->winPtr.as<int>(), ws->nwin, n,
           p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
-          c.partition == SPTRSV_COL_PARTITION ? c.window : 0,
+          ws->uniform_w,
           ws->rlist.as<int4>(), ws->roff.as<int>());
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         st = SPTRSV_ERR_HIP;

@@ -77,6 +77,38 @@ struct LaneArgs {
 
 constexpr int kPF = 4;  // off-diagonal entries per column held in registers
 
+// Wave-uniform broadcast of lane i's value (v_readlane; i is uniform).
+__device__ __forceinline__ int bcast(int v, int i) {
+  return __builtin_amdgcn_readlane(v, i);
+}
+__device__ __forceinline__ double bcast(double v, int i) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, i);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), i);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float bcast(float v, int i) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
+}
+
+// Ready lanes at which a wavefront stops sweeping one lane at a time and
+// processes them all in parallel instead.
+constexpr int kParallelReady = 16;
+
+// Compute wavefront cw handles batches of 64 consecutive columns (lane =
+// column) cw, cw + ncw, ...  Per refresh it reads every lane's LDS arrival
+// count and partial value once.  If many lanes are ready (independent
+// columns) they are solved in parallel and publish through LDS.  Otherwise
+// the wavefront sweeps the ready lanes in ascending order: the lowest ready
+// lane's x and its first kPF entries are broadcast with v_readlane, and every
+// lane that is the target of such an entry inside this batch adds it to
+// registers at once, so a dependency chain inside the batch costs a few
+// scalar/vector instructions per hop instead of an LDS round trip.  The swept
+// lanes publish their remaining entries (LDS for the window, slots beyond it)
+// together at the end of the sweep.  Every entry is delivered exactly once:
+// by register when it is a sweep-time in-batch entry, otherwise by LDS/slot,
+// and a lane is ready when LDS arrivals + register arrivals reach its total.
 template <typename T, bool BWD>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int wlo, int wlen,
@@ -87,10 +119,11 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   for (int base = cw * kWave; base < wlen; base += ncw * kWave) {
     const int k = base + lane;
     bool active = k < wlen;
-    int lo = 0, hi = 0, total = 0;
+    int lo = 0, hi = 0, total = 0, ne = 0;
     T diag = T(1), inv = T(1);
     int tg[kPF];
     T vv[kPF];
+    unsigned inb = 0;  // bit e: entry e targets a column of this batch
     if (active) {
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
@@ -99,54 +132,104 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       inv = T(1) / diag;  // off the critical path
       lo = BWD ? f : f + 1;
       hi = BWD ? l - 1 : l;
+      ne = min(hi - lo, kPF);
       const int2 nd = a.need[c];
       total = nd.x + nd.y;
 #pragma unroll
       for (int i = 0; i < kPF; i++) {
         const int j = lo + i;
-        tg[i] = j < hi ? a.etgt[j] : -1;
+        tg[i] = j < hi ? a.etgt[j] : 0;
         vv[i] = j < hi ? a.val[j] : T(0);
+        // targets lie after the column, so in-batch means below base + 64
+        if (i < ne && tg[i] < 0 && -tg[i] - 1 < base + kWave) inb |= 1u << i;
       }
     }
+    T racc = T(0);  // in-batch contributions received by register
+    int rcnt = 0;   // ... and their count
     StallWatch watch;
     while (__ballot(active) != 0) {
-      bool progressed = false;
-      if (active && ld_wg(&s_cnt[k]) == total) {
-        fence_acq_wg();
-        // quotient by the precomputed reciprocal plus one residual correction
-        // (two fused multiply-adds instead of a full division on the chain)
-        const T q = s_val[k];
-        T xv = q * inv;
-        xv = __builtin_fma(__builtin_fma(-xv, diag, q), inv, xv);
-        s_val[k] = xv;  // the column is final: its slot now holds x
-#pragma unroll
-        for (int i = 0; i < kPF; i++)
-          if (lo + i < hi && tg[i] < 0) add_wg(&s_val[-tg[i] - 1], -(xv * vv[i]));
-        for (int j = lo + kPF; j < hi; j++) {
-          const int t = a.etgt[j];
-          if (t < 0) add_wg(&s_val[-t - 1], -(xv * a.val[j]));
+      const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
+      fence_acq_wg();
+      const T val = active ? s_val[k] : T(0);
+      bool rdy = active && cnt + rcnt == total;
+      unsigned long long m = __ballot(rdy);
+      const bool any = m != 0;
+      bool solved = false;    // solved in this refresh
+      bool swept = false;     // ... by the sweep (in-batch entries already sent)
+      T xv = T(0);
+      if (__popcll(m) >= kParallelReady) {
+        if (rdy) {
+          const T q = val - racc;
+          xv = q * inv;
+          xv = __builtin_fma(__builtin_fma(-xv, diag, q), inv, xv);
+          solved = true;
+          active = false;
         }
-        fence_rel_wg();
+      } else {
+        while (m != 0) {
+          const int i = __builtin_ctzll(m);
+          // every lane evaluates its quotient; lane i's is final
+          const T q = val - racc;
+          T xx = q * inv;
+          xx = __builtin_fma(__builtin_fma(-xx, diag, q), inv, xx);
+          if (lane == i) {
+            xv = xx;
+            solved = true;
+            swept = true;
+            active = false;
+          }
+          const T xi = bcast(xx, i);
+          const unsigned bi = (unsigned)bcast((int)inb, i);
 #pragma unroll
-        for (int i = 0; i < kPF; i++) {
-          if (lo + i < hi) {
-            if (tg[i] < 0)
-              add_wg(&s_cnt[-tg[i] - 1], 1);
-            else
-              xchg_agent(&slots[tg[i]], xv * vv[i]);
+          for (int e = 0; e < kPF; e++) {
+            if ((bi >> e) & 1u) {
+              const int te = bcast(tg[e], i);
+              const T ve = bcast(vv[e], i);
+              if (active && te == -k - 1) {
+                racc += xi * ve;
+                rcnt += 1;
+              }
+            }
+          }
+          rdy = active && cnt + rcnt == total;
+          m = __ballot(rdy);
+        }
+      }
+      if (__ballot(solved) != 0) {
+        // publish: sums before counters (LDS), slots beyond the window
+        const unsigned viaLds = swept ? ~inb : ~0u;
+        if (solved) {
+          s_val[k] = xv;  // the column is final: its LDS value now holds x
+#pragma unroll
+          for (int e = 0; e < kPF; e++)
+            if (e < ne && ((viaLds >> e) & 1u) && tg[e] < 0)
+              add_wg(&s_val[-tg[e] - 1], -(xv * vv[e]));
+          for (int j = lo + kPF; j < hi; j++) {
+            const int t = a.etgt[j];
+            if (t < 0) add_wg(&s_val[-t - 1], -(xv * a.val[j]));
           }
         }
-        for (int j = lo + kPF; j < hi; j++) {
-          const int t = a.etgt[j];
-          if (t < 0)
-            add_wg(&s_cnt[-t - 1], 1);
-          else
-            xchg_agent(&slots[t], xv * a.val[j]);
+        fence_rel_wg();
+        if (solved) {
+#pragma unroll
+          for (int e = 0; e < kPF; e++) {
+            if (e < ne && ((viaLds >> e) & 1u)) {
+              if (tg[e] < 0)
+                add_wg(&s_cnt[-tg[e] - 1], 1);
+              else
+                xchg_agent(&slots[tg[e]], xv * vv[e]);
+            }
+          }
+          for (int j = lo + kPF; j < hi; j++) {
+            const int t = a.etgt[j];
+            if (t < 0)
+              add_wg(&s_cnt[-t - 1], 1);
+            else
+              xchg_agent(&slots[t], xv * a.val[j]);
+          }
         }
-        active = false;
-        progressed = true;
       }
-      if (watch.step(__ballot(progressed) != 0, a.err, a.wait_cap)) active = false;
+      if (watch.step(any, a.err, a.wait_cap)) active = false;
     }
   }
 }
