@@ -147,13 +147,21 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     T racc = T(0);  // in-batch contributions received by register
     int rcnt = 0;   // ... and their count
     StallWatch watch;
+    int level = 0;
     while (__ballot(active) != 0) {
       const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
-      fence_acq_wg();
-      const T val = active ? s_val[k] : T(0);
       bool rdy = active && cnt + rcnt == total;
       unsigned long long m = __ballot(rdy);
       const bool any = m != 0;
+      // nothing ready: short back-off, so idle wavefronts leave LDS bandwidth
+      // to the ones that are solving
+      backoff_short(level, any);
+      if (!any) {
+        if (watch.step(false, a.err, a.wait_cap)) active = false;
+        continue;
+      }
+      fence_acq_wg();
+      const T val = active ? s_val[k] : T(0);
       bool solved = false;    // solved in this refresh
       bool swept = false;     // ... by the sweep (in-batch entries already sent)
       T xv = T(0);
