@@ -124,6 +124,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     int tg[kPF];
     T vv[kPF];
     unsigned inb = 0;  // bit e: entry e targets a column of this batch
+    int tl[kPF];       // ... and that column's lane
     if (active) {
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
@@ -140,9 +141,13 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         const int j = lo + i;
         tg[i] = j < hi ? a.etgt[j] : 0;
         vv[i] = j < hi ? a.val[j] : T(0);
-        // targets lie after the column, so in-batch means below base + 64
-        if (i < ne && tg[i] < 0 && -tg[i] - 1 < base + kWave) inb |= 1u << i;
       }
+    }
+#pragma unroll
+    for (int i = 0; i < kPF; i++) {
+      // targets lie after the column, so in-batch means below base + 64
+      tl[i] = -tg[i] - 1 - base;
+      if (active && i < ne && tg[i] < 0 && tl[i] < kWave) inb |= 1u << i;
     }
     T racc = T(0);  // in-batch contributions received by register
     int rcnt = 0;   // ... and their count
@@ -180,28 +185,26 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           const T q = val - racc;
           T xx = q * inv;
           xx = __builtin_fma(__builtin_fma(-xx, diag, q), inv, xx);
-          if (lane == i) {
-            xv = xx;
-            solved = true;
-            swept = true;
-            active = false;
-          }
+          const bool me = lane == i;
+          xv = me ? xx : xv;
+          solved = solved || me;
+          active = active && !me;
           const T xi = bcast(xx, i);
           const unsigned bi = (unsigned)bcast((int)inb, i);
 #pragma unroll
           for (int e = 0; e < kPF; e++) {
-            if ((bi >> e) & 1u) {
-              const int te = bcast(tg[e], i);
+            if ((bi >> e) & 1u) {  // uniform
+              const int t = bcast(tl[e], i);
               const T ve = bcast(vv[e], i);
-              if (active && te == -k - 1) {
-                racc += xi * ve;
-                rcnt += 1;
-              }
+              const bool hit = lane == t;  // the target is never solved yet
+              racc = hit ? __builtin_fma(xi, ve, racc) : racc;
+              rcnt += hit ? 1 : 0;
             }
           }
           rdy = active && cnt + rcnt == total;
           m = __ballot(rdy);
         }
+        swept = solved;
       }
       if (__ballot(solved) != 0) {
         // publish: sums before counters (LDS), slots beyond the window
