@@ -95,6 +95,8 @@ __device__ __forceinline__ float bcast(float v, int i) {
 // Ready lanes at which a wavefront stops sweeping one lane at a time and
 // processes them all in parallel instead.
 constexpr int kParallelReady = 16;
+// Lanes a sweep solves between two publications of their results.
+constexpr int kPublish = 16;
 
 // Compute wavefront cw handles batches of 64 consecutive columns (lane =
 // column) cw, cw + ncw, ...  Per refresh it reads every lane's LDS arrival
@@ -167,47 +169,11 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       }
       fence_acq_wg();
       const T val = active ? s_val[k] : T(0);
-      bool solved = false;    // solved in this refresh
+      bool solved = false;    // solved, not yet published
       bool swept = false;     // ... by the sweep (in-batch entries already sent)
       T xv = T(0);
-      if (__popcll(m) >= kParallelReady) {
-        if (rdy) {
-          const T q = val - racc;
-          xv = q * inv;
-          xv = __builtin_fma(__builtin_fma(-xv, diag, q), inv, xv);
-          solved = true;
-          active = false;
-        }
-      } else {
-        while (m != 0) {
-          const int i = __builtin_ctzll(m);
-          // every lane evaluates its quotient; lane i's is final
-          const T q = val - racc;
-          T xx = q * inv;
-          xx = __builtin_fma(__builtin_fma(-xx, diag, q), inv, xx);
-          const bool me = lane == i;
-          xv = me ? xx : xv;
-          solved = solved || me;
-          active = active && !me;
-          const T xi = bcast(xx, i);
-          const unsigned bi = (unsigned)bcast((int)inb, i);
-#pragma unroll
-          for (int e = 0; e < kPF; e++) {
-            if ((bi >> e) & 1u) {  // uniform
-              const int t = bcast(tl[e], i);
-              const T ve = bcast(vv[e], i);
-              const bool hit = lane == t;  // the target is never solved yet
-              racc = hit ? __builtin_fma(xi, ve, racc) : racc;
-              rcnt += hit ? 1 : 0;
-            }
-          }
-          rdy = active && cnt + rcnt == total;
-          m = __ballot(rdy);
-        }
-        swept = solved;
-      }
-      if (__ballot(solved) != 0) {
-        // publish: sums before counters (LDS), slots beyond the window
+      // publish the solved lanes: LDS sums, then LDS counters and slots
+      auto publish = [&]() {
         const unsigned viaLds = swept ? ~inb : ~0u;
         if (solved) {
           s_val[k] = xv;  // the column is final: its LDS value now holds x
@@ -239,6 +205,51 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
               xchg_agent(&slots[t], xv * a.val[j]);
           }
         }
+        solved = false;
+      };
+      if (__popcll(m) >= kParallelReady) {
+        if (rdy) {
+          const T q = val - racc;
+          xv = q * inv;
+          xv = __builtin_fma(__builtin_fma(-xv, diag, q), inv, xv);
+          solved = true;
+          active = false;
+        }
+        publish();
+      } else {
+        int since = 0;
+        while (m != 0) {
+          const int i = __builtin_ctzll(m);
+          // every lane evaluates its quotient; lane i's is final
+          const T q = val - racc;
+          T xx = q * inv;
+          xx = __builtin_fma(__builtin_fma(-xx, diag, q), inv, xx);
+          const bool me = lane == i;
+          xv = me ? xx : xv;
+          solved = solved || me;
+          swept = swept || me;
+          active = active && !me;
+          const T xi = bcast(xx, i);
+          const unsigned bi = (unsigned)bcast((int)inb, i);
+#pragma unroll
+          for (int e = 0; e < kPF; e++) {
+            if ((bi >> e) & 1u) {  // uniform
+              const int t = bcast(tl[e], i);
+              const T ve = bcast(vv[e], i);
+              const bool hit = lane == t;  // the target is never solved yet
+              racc = hit ? __builtin_fma(xi, ve, racc) : racc;
+              rcnt += hit ? 1 : 0;
+            }
+          }
+          rdy = active && cnt + rcnt == total;
+          m = __ballot(rdy);
+          // hand results to other batches and windows every kPublish lanes
+          if (++since == kPublish && m != 0) {
+            publish();
+            since = 0;
+          }
+        }
+        if (__ballot(solved) != 0) publish();
       }
       if (watch.step(any, a.err, a.wait_cap)) active = false;
     }
