@@ -414,9 +414,9 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
     c.lanes_per_column = 1;
     c.rhs_lanes = 1;
     c.rhs_per_lane = 1;
-    c.block = 256;
-    c.window = 4096;
-    if ((size_t)c.window * 12 > p->lds_max) c.window = 2048;
+    c.block = 512;
+    c.window = 8192;
+    while (c.window > 1024 && (size_t)c.window * 12 > p->lds_max) c.window /= 2;
   } else {
     int rl = 1;
     while (rl < rhs && rl < 64) rl <<= 1;
@@ -1049,8 +1049,8 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     c.partition = part;
     if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
   };
-  for (int block : {256, 512, 1024})
-    for (int window : {1024, 2048, 4096, 8192})
+  for (int block : {512, 1024, 256})
+    for (int window : {8192, 4096, 2048})
       add(1, 1, 1, block, window, 0, SPTRSV_ENGINE_LANE);
   if (rhs <= 1) {
     for (int G : {1, 2, 4, 8, 16, 64})
