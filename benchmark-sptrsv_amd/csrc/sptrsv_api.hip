@@ -377,6 +377,7 @@ struct WindowSet {
   DevBuf rptr;    // int[n+1]
   DevBuf etgt;    // int[nnz]
   DevBuf rlist;   // int4[#columns with remote inputs]
+  DevBuf chain;   // int2[n]: in-batch predecessor delivered by register
   DevBuf roff;    // int[nwin+1]
   int nslots = -1;
 };
@@ -658,6 +659,34 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     cursor.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
   }
+  // in-batch predecessors delivered by register (must follow k_lane_targets,
+  // whose etgt values it overwrites for the selected entries)
+  ST_TRY(ws->chain.ensure(sizeof(int2) * (size_t)std::max(n, 1), false));
+  if (n > 0) {
+    DevBuf key;
+    ST_TRY(key.ensure(sizeof(unsigned long long) * (size_t)n, true));
+    const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
+    if (p->vt == SPTRSV_F64)
+      k_chain_key<double><<<grid_for(n, 256), 256, 0, s>>>(
+          p->colPtr, p->rowIdx, (const double *)p->val, n, bwd,
+          ws->winPtr.as<int>(), ws->nwin, ws->uniform_w,
+          key.as<unsigned long long>());
+    else
+      k_chain_key<float><<<grid_for(n, 256), 256, 0, s>>>(
+          p->colPtr, p->rowIdx, (const float *)p->val, n, bwd,
+          ws->winPtr.as<int>(), ws->nwin, ws->uniform_w,
+          key.as<unsigned long long>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+      k_chain_mark<<<grid_for(n, 256), 256, 0, s>>>(
+          key.as<unsigned long long>(), n, bwd, ws->winPtr.as<int>(), ws->nwin,
+          ws->uniform_w, ws->etgt.as<int>(), ws->chain.as<int2>());
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    key.release();
+    if (e != hipSuccess) return SPTRSV_ERR_HIP;
+  }
   // receive lists: the columns with remote inputs, grouped by window
   {
     DevBuf pos;
@@ -802,6 +831,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.val = val;
     la.etgt = L.ws->etgt.as<int>();
     la.need = L.ws->need.as<int2>();
+    la.chain = L.ws->chain.as<int2>();
     la.rptr = L.ws->rptr.as<int>();
     la.winPtr = L.ws->winPtr.as<int>();
     la.rlist = L.ws->rlist.as<int4>();
@@ -1386,6 +1416,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     kv.second->etgt.release();
     kv.second->rlist.release();
     kv.second->roff.release();
+    kv.second->chain.release();
     delete kv.second;
   }
   p->windows.clear();
@@ -1501,7 +1532,8 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
   for (auto &kv : p->windows)
     bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes +
                          kv.second->rptr.bytes + kv.second->etgt.bytes +
-                         kv.second->rlist.bytes + kv.second->roff.bytes);
+                         kv.second->rlist.bytes + kv.second->roff.bytes +
+                         kv.second->chain.bytes);
   info->scratch_bytes = bytes;
   info->analysis_ms = p->analysis_ms;
   return SPTRSV_OK;

@@ -60,6 +60,8 @@ struct LaneArgs {
   const T *__restrict__ val;
   const int *__restrict__ etgt;   // [nnz]: local -(lk+1) or slot index
   const int2 *__restrict__ need;  // [n] by row: {local, remote} arrivals
+  const int2 *__restrict__ chain; // [n] by row: {in-batch predecessor lane
+                                  // or -1, its entry index}
   const int *__restrict__ rptr;   // [n+1] slot range of each row
   const int *__restrict__ winPtr; // [nwin + 1] window bounds, solve order
   const int4 *__restrict__ rlist; // columns with remote inputs, window order:
@@ -77,40 +79,26 @@ struct LaneArgs {
 
 constexpr int kPF = 4;  // off-diagonal entries per column held in registers
 
-// Wave-uniform broadcast of lane i's value (v_readlane; i is uniform).
-__device__ __forceinline__ int bcast(int v, int i) {
-  return __builtin_amdgcn_readlane(v, i);
-}
-__device__ __forceinline__ double bcast(double v, int i) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, i);
-  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), i);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-__device__ __forceinline__ float bcast(float v, int i) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
-}
-
-// Ready lanes at which a wavefront stops sweeping one lane at a time and
-// processes them all in parallel instead.
-constexpr int kParallelReady = 16;
-// Lanes a sweep solves between two publications of their results.
-constexpr int kPublish = 16;
+// An entry delivered inside its wavefront instead of through LDS or a slot
+// (etgt value): the nearest earlier column of the same 64-column batch that
+// row r depends on, when |L(r, c)| <= |L(r, r)|.
+constexpr int kRegTarget = -2147483647 - 1;
 
 // Compute wavefront cw handles batches of 64 consecutive columns (lane =
-// column) cw, cw + ncw, ...  Per refresh it reads every lane's LDS arrival
-// count and partial value once.  If many lanes are ready (independent
-// columns) they are solved in parallel and publish through LDS.  Otherwise
-// the wavefront sweeps the ready lanes in ascending order: the lowest ready
-// lane's x and its first kPF entries are broadcast with v_readlane, and every
-// lane that is the target of such an entry inside this batch adds it to
-// registers at once, so a dependency chain inside the batch costs a few
-// scalar/vector instructions per hop instead of an LDS round trip.  The swept
-// lanes publish their remaining entries (LDS for the window, slots beyond it)
-// together at the end of the sweep.  Every entry is delivered exactly once:
-// by register when it is a sweep-time in-batch entry, otherwise by LDS/slot,
-// and a lane is ready when LDS arrivals + register arrivals reach its total.
+// column) cw, cw + ncw, ...  Each row r may have one in-batch predecessor
+// p(r) whose entry is delivered by register (analysis: `chain`); every other
+// input arrives through LDS (same window) or a slot (earlier window, folded
+// into LDS by the receivers).  Once a lane's LDS arrivals are complete its
+// value is affine in its predecessor's:
+//     x_r = r_r / d_r + c_r * x_p(r),   c_r = -L(r, p(r)) / d_r,  |c_r| <= 1,
+// so the lanes whose inputs are complete along their whole predecessor chain
+// are solved together by pointer jumping over the batch's forest
+// (log2(chain depth) rounds of cross-lane permutes), instead of one lane per
+// step.  Solved lanes publish at once: LDS partial values, then (after a
+// workgroup release) LDS arrival counters; slots are written first, since a
+// slot's value is its own arrival flag.  Lanes whose chain is not complete
+// wait for the next round; the lowest unsolved lane always completes once its
+// inputs from earlier batches arrive, so the batch always drains.
 template <typename T, bool BWD>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int wlo, int wlen,
@@ -118,15 +106,14 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   const int lane = __lane_id();
   const int n = a.n;
   T *slots = a.slots + (size_t)tile * a.nslots;
-  for (int base = cw * kWave; base < wlen; base += ncw * kWave) {
+  bool stopped = false;  // the stall watch fired: leave the window
+  for (int base = cw * kWave; base < wlen && !stopped; base += ncw * kWave) {
     const int k = base + lane;
     bool active = k < wlen;
-    int lo = 0, hi = 0, total = 0, ne = 0;
-    T diag = T(1), inv = T(1);
+    int lo = 0, hi = 0, need_lds = 0, ne = 0, pl = lane;
+    T diag = T(1), inv = T(1), coef = T(0);
     int tg[kPF];
     T vv[kPF];
-    unsigned inb = 0;  // bit e: entry e targets a column of this batch
-    int tl[kPF];       // ... and that column's lane
     if (active) {
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
@@ -137,121 +124,101 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       hi = BWD ? l - 1 : l;
       ne = min(hi - lo, kPF);
       const int2 nd = a.need[c];
-      total = nd.x + nd.y;
+      const int2 ch = a.chain[c];
+      need_lds = nd.x + nd.y;
+      if (ch.x >= 0 && ch.x < lane) {
+        pl = ch.x;
+        need_lds -= 1;
+        const T lv = a.val[ch.y];
+        T cq = lv * inv;
+        cq = __builtin_fma(__builtin_fma(-cq, diag, lv), inv, cq);
+        coef = -cq;
+      }
 #pragma unroll
       for (int i = 0; i < kPF; i++) {
         const int j = lo + i;
-        tg[i] = j < hi ? a.etgt[j] : 0;
+        tg[i] = j < hi ? a.etgt[j] : kRegTarget;
         vv[i] = j < hi ? a.val[j] : T(0);
       }
     }
-#pragma unroll
-    for (int i = 0; i < kPF; i++) {
-      // targets lie after the column, so in-batch means below base + 64
-      tl[i] = -tg[i] - 1 - base;
-      if (active && i < ne && tg[i] < 0 && tl[i] < kWave) inb |= 1u << i;
-    }
-    T racc = T(0);  // in-batch contributions received by register
-    int rcnt = 0;   // ... and their count
+    T xs = T(0);  // x of the lanes solved in earlier rounds
     StallWatch watch;
     int level = 0;
     while (__ballot(active) != 0) {
       const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
-      bool rdy = active && cnt + rcnt == total;
-      unsigned long long m = __ballot(rdy);
-      const bool any = m != 0;
-      // nothing ready: short back-off, so idle wavefronts leave LDS bandwidth
-      // to the ones that are solving
-      backoff_short(level, any);
-      if (!any) {
-        if (watch.step(false, a.err, a.wait_cap)) active = false;
-        continue;
-      }
-      fence_acq_wg();
-      const T val = active ? s_val[k] : T(0);
-      bool solved = false;    // solved, not yet published
-      bool swept = false;     // ... by the sweep (in-batch entries already sent)
-      T xv = T(0);
-      // publish the solved lanes: LDS sums, then LDS counters and slots
-      auto publish = [&]() {
-        const unsigned viaLds = swept ? ~inb : ~0u;
-        if (solved) {
-          s_val[k] = xv;  // the column is final: its LDS value now holds x
-#pragma unroll
-          for (int e = 0; e < kPF; e++)
-            if (e < ne && ((viaLds >> e) & 1u) && tg[e] < 0)
-              add_wg(&s_val[-tg[e] - 1], -(xv * vv[e]));
-          for (int j = lo + kPF; j < hi; j++) {
-            const int t = a.etgt[j];
-            if (t < 0) add_wg(&s_val[-t - 1], -(xv * a.val[j]));
-          }
+      const bool ext = active && cnt == need_lds;
+      bool progressed = false;  // some lane solved in this round
+      if (__ballot(ext) != 0) {
+        fence_acq_wg();
+        const T r = ext ? s_val[k] : T(0);
+        T q = r * inv;
+        q = __builtin_fma(__builtin_fma(-q, diag, r), inv, q);
+        // (av, cf, pp): x = av + cf * x_pp; solved lanes are roots (cf = 0)
+        T av = active ? q : xs;
+        T cf = active ? coef : T(0);
+        int pp = active ? pl : lane;
+        bool ok = active ? ext : true;
+        // Every lane runs the rounds, so every permute reads a live lane.  A
+        // lane is done once it has absorbed a root (cf becomes 0): at most 6
+        // rounds for a 64-lane batch.  A product of coefficients that
+        // underflows to 0 also ends a lane; the terms it leaves out are below
+        // the smallest normal number times the ancestors' values.
+        while (__ballot(ok && cf != T(0)) != 0) {
+          const int qk = __shfl(pp | (ok ? 0 : 64), pp);
+          const T ap = __shfl(av, pp);
+          const T cp = __shfl(cf, pp);
+          av = __builtin_fma(cf, ap, av);
+          cf = cf * cp;
+          ok = ok && (qk & 64) == 0;
+          pp = qk & 63;
         }
-        fence_rel_wg();
+        const bool solved = active && ok;
         if (solved) {
-#pragma unroll
-          for (int e = 0; e < kPF; e++) {
-            if (e < ne && ((viaLds >> e) & 1u)) {
-              if (tg[e] < 0)
-                add_wg(&s_cnt[-tg[e] - 1], 1);
-              else
-                xchg_agent(&slots[tg[e]], xv * vv[e]);
-            }
-          }
-          for (int j = lo + kPF; j < hi; j++) {
-            const int t = a.etgt[j];
-            if (t < 0)
-              add_wg(&s_cnt[-t - 1], 1);
-            else
-              xchg_agent(&slots[t], xv * a.val[j]);
-          }
-        }
-        solved = false;
-      };
-      if (__popcll(m) >= kParallelReady) {
-        if (rdy) {
-          const T q = val - racc;
-          xv = q * inv;
-          xv = __builtin_fma(__builtin_fma(-xv, diag, q), inv, xv);
-          solved = true;
+          xs = av;
           active = false;
         }
-        publish();
-      } else {
-        int since = 0;
-        while (m != 0) {
-          const int i = __builtin_ctzll(m);
-          // every lane evaluates its quotient; lane i's is final
-          const T q = val - racc;
-          T xx = q * inv;
-          xx = __builtin_fma(__builtin_fma(-xx, diag, q), inv, xx);
-          const bool me = lane == i;
-          xv = me ? xx : xv;
-          solved = solved || me;
-          swept = swept || me;
-          active = active && !me;
-          const T xi = bcast(xx, i);
-          const unsigned bi = (unsigned)bcast((int)inb, i);
+        progressed = __ballot(solved) != 0;
+        if (progressed) {
+          const T xv = av;
+          if (solved) {
+            // slots first: the value is its own arrival flag
 #pragma unroll
-          for (int e = 0; e < kPF; e++) {
-            if ((bi >> e) & 1u) {  // uniform
-              const int t = bcast(tl[e], i);
-              const T ve = bcast(vv[e], i);
-              const bool hit = lane == t;  // the target is never solved yet
-              racc = hit ? __builtin_fma(xi, ve, racc) : racc;
-              rcnt += hit ? 1 : 0;
+            for (int e = 0; e < kPF; e++)
+              if (e < ne && tg[e] >= 0) xchg_agent(&slots[tg[e]], xv * vv[e]);
+            s_val[k] = xv;  // the column is final: its LDS value now holds x
+#pragma unroll
+            for (int e = 0; e < kPF; e++)
+              if (e < ne && tg[e] < 0 && tg[e] != kRegTarget)
+                add_wg(&s_val[-tg[e] - 1], -(xv * vv[e]));
+            for (int j = lo + kPF; j < hi; j++) {
+              const int t = a.etgt[j];
+              if (t >= 0)
+                xchg_agent(&slots[t], xv * a.val[j]);
+              else if (t != kRegTarget)
+                add_wg(&s_val[-t - 1], -(xv * a.val[j]));
             }
           }
-          rdy = active && cnt + rcnt == total;
-          m = __ballot(rdy);
-          // hand results to other batches and windows every kPublish lanes
-          if (++since == kPublish && m != 0) {
-            publish();
-            since = 0;
+          fence_rel_wg();
+          if (solved) {
+#pragma unroll
+            for (int e = 0; e < kPF; e++)
+              if (e < ne && tg[e] < 0 && tg[e] != kRegTarget)
+                add_wg(&s_cnt[-tg[e] - 1], 1);
+            for (int j = lo + kPF; j < hi; j++) {
+              const int t = a.etgt[j];
+              if (t < 0 && t != kRegTarget) add_wg(&s_cnt[-t - 1], 1);
+            }
           }
         }
-        if (__ballot(solved) != 0) publish();
       }
-      if (watch.step(any, a.err, a.wait_cap)) active = false;
+      // nothing solved: short back-off, so idle wavefronts leave LDS
+      // bandwidth to the ones that are solving; a wavefront whose lanes have
+      // inputs but wait on an unsolved predecessor counts as stalled
+      backoff_short(level, progressed);
+      if (watch.step(progressed, a.err, a.wait_cap)) {
+        active = false;
+        stopped = true;
+      }
     }
   }
 }
@@ -409,6 +376,59 @@ __global__ void k_lane_targets(const int *__restrict__ colPtr,
       const int pr = backward ? n - 1 - r : r;
       etgt[j] = pr < whi ? -(pr - wlo) - 1 : rptr[r] + atomicAdd(&cursor[r], 1);
     }
+  }
+}
+
+// In-batch predecessor selection: for every row, the nearest earlier column
+// of the same 64-column batch (solve order) among those with |L(r, c)| <=
+// |L(r, r)|, as key = (position + 1) << 32 | entry index, kept by atomicMax
+// (key zeroed).  The bound keeps the products of the affine coefficients of
+// a jumped chain at most 1 in magnitude.
+template <typename T>
+__global__ void k_chain_key(const int *__restrict__ colPtr,
+                            const int *__restrict__ rowIdx,
+                            const T *__restrict__ val, int n, int backward,
+                            const int *__restrict__ winPtr, int nwin,
+                            int uniform_w, unsigned long long *key) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += gridDim.x * blockDim.x) {
+    const int p = backward ? n - 1 - c : c;
+    const int w = uniform_w > 0 ? p / uniform_w
+                                : upper_bound_minus1(winPtr, nwin + 1, p);
+    const int wlo = winPtr[w], whi = winPtr[w + 1];
+    const int lb = (p - wlo) >> 6;
+    const int f = colPtr[c], l = colPtr[c + 1];
+    const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
+    for (int j = lo; j < hi; j++) {
+      const int r = rowIdx[j];
+      const int pr = backward ? n - 1 - r : r;
+      if (pr >= whi || ((pr - wlo) >> 6) != lb) continue;
+      const T d = val[backward ? colPtr[r + 1] - 1 : colPtr[r]];
+      const T v = val[j];
+      if (!(fabs(v) <= fabs(d))) continue;
+      atomicMax(&key[r], ((unsigned long long)(p + 1) << 32) | (unsigned)j);
+    }
+  }
+}
+
+// chain[r] = {lane of the selected predecessor, its entry} or {-1, 0}; the
+// selected entry's target becomes kRegTarget.
+__global__ void k_chain_mark(const unsigned long long *__restrict__ key, int n,
+                             int backward, const int *__restrict__ winPtr,
+                             int nwin, int uniform_w, int *etgt, int2 *chain) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += gridDim.x * blockDim.x) {
+    const unsigned long long kk = key[r];
+    if (kk == 0) {
+      chain[r] = make_int2(-1, 0);
+      continue;
+    }
+    const int pc = (int)(kk >> 32) - 1;
+    const int j = (int)(unsigned)(kk & 0xffffffffull);
+    const int w = uniform_w > 0 ? pc / uniform_w
+                                : upper_bound_minus1(winPtr, nwin + 1, pc);
+    chain[r] = make_int2((pc - winPtr[w]) & (kWave - 1), j);
+    etgt[j] = kRegTarget;
   }
 }
 

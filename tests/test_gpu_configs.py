@@ -1,8 +1,10 @@
 """Parity at the benchmark configurations' full sizes.
 
-cfg1  2D 5-point Laplacian 256^2, x = 1: known answer, bit-exact.
+cfg1  2D 5-point Laplacian 256^2, x = 1: known answer within KNOWN_ANSWER_TOL
+      (the default lane engine reassociates in-batch chains; see
+      test_gpu_parity.py).
 cfg2  3D 7-point Laplacian 128^3: FP64 <= 1e-12 vs the serial oracle; x = 1
-      known answer bit-exact; FP32 + FP64 refinement <= 1e-10.
+      known answer within KNOWN_ANSWER_TOL; FP32 + FP64 refinement <= 1e-10.
 cfg3  power-law n = 1e6: FP64 <= 1e-12, refinement <= 1e-10.
 cfg4  cfg2 matrix with 8 / 32 / 128 right-hand sides: FP64 <= 1e-12,
       refinement <= 1e-10.
@@ -15,6 +17,8 @@ import pytest
 import sptrsv_amd as S
 from sptrsv_amd import matrices as mx
 from gpu_util import assert_close, oracle_x, plan_solve, require_gpu
+
+KNOWN_ANSWER_TOL = 1e-14
 
 pytestmark = pytest.mark.gpu
 
@@ -42,7 +46,7 @@ def test_cfg1_known_answer():
     f = mx.laplacian2d_lower(256, 256)
     b = f.spmv(np.ones(f.n))
     x, _ = plan_solve(f, b)
-    assert np.array_equal(x, np.ones(f.n))
+    assert np.abs(x - 1.0).max() <= KNOWN_ANSWER_TOL
 
 
 def test_cfg2_fp64_and_known_answer(lap3d):
@@ -52,7 +56,7 @@ def test_cfg2_fp64_and_known_answer(lap3d):
     assert_close(x, oracle_x(f, b), 1e-12)
     b1 = f.spmv(np.ones(f.n))
     x1, _ = plan_solve(f, b1, plan=plan)
-    assert np.array_equal(x1, np.ones(f.n))
+    assert np.abs(x1 - 1.0).max() <= KNOWN_ANSWER_TOL
 
 
 def test_cfg2_refinement(lap3d):
