@@ -407,26 +407,17 @@ struct sptrsv_plan_s {
 namespace {
 
 sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
+  (void)rhs;  // the lane engine runs one tile per right-hand side
   sptrsv_config_t c;
   memset(&c, 0, sizeof(c));
-  if (rhs <= 1) {
-    // lane engine: one lane per column, 4096-column LDS windows
-    c.engine = SPTRSV_ENGINE_LANE;
-    c.lanes_per_column = 1;
-    c.rhs_lanes = 1;
-    c.rhs_per_lane = 1;
-    c.block = 512;
-    c.window = 8192;
-    while (c.window > 1024 && (size_t)c.window * 12 > p->lds_max) c.window /= 2;
-  } else {
-    int rl = 1;
-    while (rl < rhs && rl < 64) rl <<= 1;
-    c.lanes_per_column = rl;
-    c.rhs_lanes = rl;
-    c.rhs_per_lane = rhs > 64 ? std::min(4, (rhs + 63) / 64) : 1;
-    c.block = 256;
-    c.window = std::max(1, 256 / rl);
-  }
+  // lane engine: one lane per column, 8192-column LDS windows, 16 wavefronts
+  c.engine = SPTRSV_ENGINE_LANE;
+  c.lanes_per_column = 1;
+  c.rhs_lanes = 1;
+  c.rhs_per_lane = 1;
+  c.block = 1024;
+  c.window = 8192;
+  while (c.window > 1024 && (size_t)c.window * 12 > p->lds_max) c.window /= 2;
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
   return c;
@@ -1079,7 +1070,7 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     c.partition = part;
     if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
   };
-  for (int block : {512, 1024, 256})
+  for (int block : {1024, 512, 256})
     for (int window : {8192, 4096, 2048})
       add(1, 1, 1, block, window, 0, SPTRSV_ENGINE_LANE);
   if (rhs <= 1) {

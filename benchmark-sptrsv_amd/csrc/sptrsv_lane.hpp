@@ -79,6 +79,54 @@ struct LaneArgs {
 
 constexpr int kPF = 4;  // off-diagonal entries per column held in registers
 
+// Cross-lane move with DPP (no LDS): lanes whose source is outside the
+// row or whose row is masked off get `old`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_t(double old, double v) {
+  const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+  const unsigned long long s = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)dpp_i<CTRL, ROWMASK>((int)(unsigned)o, (int)(unsigned)s);
+  const unsigned hi =
+      (unsigned)dpp_i<CTRL, ROWMASK>((int)(unsigned)(o >> 32), (int)(unsigned)(s >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_t(float old, float v) {
+  return __builtin_bit_cast(
+      float, dpp_i<CTRL, ROWMASK>(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v)));
+}
+
+// One step of an inclusive wavefront scan of affine maps x -> a + c * x_prev
+// with a readiness flag g: the element from the earlier lane (a1, c1, g1)
+// composes into this one as (a + c a1, c c1, g && (c == 0 || g1)).  The
+// identity (0, 1, 1) fills lanes without a source.
+template <typename T, int CTRL, int ROWMASK>
+__device__ __forceinline__ void scan_step(T &a, T &c, int &g) {
+  const T a1 = dpp_t<CTRL, ROWMASK>(T(0), a);
+  const T c1 = dpp_t<CTRL, ROWMASK>(T(1), c);
+  const int g1 = dpp_i<CTRL, ROWMASK>(1, g);
+  g = g & ((c == T(0) ? 1 : 0) | g1);
+  a = __builtin_fma(c, a1, a);
+  c = c * c1;
+}
+
+// Inclusive scan over the 64 lanes: row_shr 1, 2, 4, 8 inside each row of
+// 16, then row_bcast:15 into rows 1 and 3, then row_bcast:31 into rows 2
+// and 3 (the gfx9 DPP controls 0x111..0x118, 0x142, 0x143).
+template <typename T>
+__device__ __forceinline__ void chain_scan(T &a, T &c, int &g) {
+  scan_step<T, 0x111, 0xf>(a, c, g);
+  scan_step<T, 0x112, 0xf>(a, c, g);
+  scan_step<T, 0x114, 0xf>(a, c, g);
+  scan_step<T, 0x118, 0xf>(a, c, g);
+  scan_step<T, 0x142, 0xa>(a, c, g);
+  scan_step<T, 0x143, 0xc>(a, c, g);
+}
+
 // An entry delivered inside its wavefront instead of through LDS or a slot
 // (etgt value): the nearest earlier column of the same 64-column batch that
 // row r depends on, when |L(r, c)| <= |L(r, r)|.
@@ -141,6 +189,10 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         vv[i] = j < hi ? a.val[j] : T(0);
       }
     }
+    // every predecessor link is lane - 1 (stencil rows, chains): scan with
+    // DPP instead of pointer jumping through LDS permutes
+    const bool chain_batch =
+        __ballot(active && pl != lane && pl != lane - 1) == 0;
     T xs = T(0);  // x of the lanes solved in earlier rounds
     StallWatch watch;
     int level = 0;
@@ -156,21 +208,29 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         // (av, cf, pp): x = av + cf * x_pp; solved lanes are roots (cf = 0)
         T av = active ? q : xs;
         T cf = active ? coef : T(0);
-        int pp = active ? pl : lane;
         bool ok = active ? ext : true;
-        // Every lane runs the rounds, so every permute reads a live lane.  A
-        // lane is done once it has absorbed a root (cf becomes 0): at most 6
-        // rounds for a 64-lane batch.  A product of coefficients that
-        // underflows to 0 also ends a lane; the terms it leaves out are below
-        // the smallest normal number times the ancestors' values.
-        while (__ballot(ok && cf != T(0)) != 0) {
-          const int qk = __shfl(pp | (ok ? 0 : 64), pp);
-          const T ap = __shfl(av, pp);
-          const T cp = __shfl(cf, pp);
-          av = __builtin_fma(cf, ap, av);
-          cf = cf * cp;
-          ok = ok && (qk & 64) == 0;
-          pp = qk & 63;
+        if (chain_batch) {
+          // lane 0 never has an in-batch predecessor, so after the full
+          // scan every cf is 0 and av is x wherever g holds
+          int g = ok ? 1 : 0;
+          chain_scan<T>(av, cf, g);
+          ok = g != 0;
+        } else {
+          int pp = active ? pl : lane;
+          // Every lane runs the rounds, so every permute reads a live lane.
+          // A lane is done once it has absorbed a root (cf becomes 0): at
+          // most 6 rounds for a 64-lane batch.  A product of coefficients
+          // that underflows to 0 also ends a lane; the terms it leaves out
+          // are below the smallest normal number times the ancestors' values.
+          while (__ballot(ok && cf != T(0)) != 0) {
+            const int qk = __shfl(pp | (ok ? 0 : 64), pp);
+            const T ap = __shfl(av, pp);
+            const T cp = __shfl(cf, pp);
+            av = __builtin_fma(cf, ap, av);
+            cf = cf * cp;
+            ok = ok && (qk & 64) == 0;
+            pp = qk & 63;
+          }
         }
         const bool solved = active && ok;
         if (solved) {
