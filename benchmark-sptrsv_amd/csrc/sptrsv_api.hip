@@ -407,17 +407,20 @@ struct sptrsv_plan_s {
 namespace {
 
 sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
-  (void)rhs;  // the lane engine runs one tile per right-hand side
   sptrsv_config_t c;
   memset(&c, 0, sizeof(c));
-  // lane engine: one lane per column, 8192-column LDS windows, 16 wavefronts
+  // lane engine: one lane per column carrying up to 4 right-hand sides,
+  // the widest LDS window that fits (8192 columns at one right-hand side),
+  // 16 wavefronts
   c.engine = SPTRSV_ENGINE_LANE;
   c.lanes_per_column = 1;
   c.rhs_lanes = 1;
-  c.rhs_per_lane = 1;
+  c.rhs_per_lane = rhs >= 4 ? 4 : rhs >= 2 ? 2 : 1;
   c.block = 1024;
   c.window = 8192;
-  while (c.window > 1024 && (size_t)c.window * 12 > p->lds_max) c.window /= 2;
+  const size_t per_col = (size_t)c.rhs_per_lane * 8 + sizeof(int);
+  while (c.window > 256 && (size_t)c.window * per_col > p->lds_max)
+    c.window /= 2;
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
   return c;
@@ -425,14 +428,17 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
 
 int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
   if (c.engine == SPTRSV_ENGINE_LANE) {
-    if (c.lanes_per_column != 1 || c.rhs_lanes != 1 || c.rhs_per_lane != 1)
+    // rhs_per_lane = right-hand sides each lane carries (1, 2, 4 or 8)
+    const int R = c.rhs_per_lane;
+    if (c.lanes_per_column != 1 || c.rhs_lanes != 1 ||
+        !(R == 1 || R == 2 || R == 4 || R == 8))
       return SPTRSV_ERR_INVALID_ARG;
     if (c.block < 128 || c.block > 1024 || c.block % 64 != 0 || c.window < 1)
       return SPTRSV_ERR_INVALID_ARG;
     if (c.partition != SPTRSV_COL_PARTITION &&
         c.partition != SPTRSV_NNZ_PARTITION)
       return SPTRSV_ERR_INVALID_ARG;
-    if ((size_t)c.window * (tsize + sizeof(int)) > p->lds_max)
+    if ((size_t)c.window * ((size_t)R * tsize + sizeof(int)) > p->lds_max)
       return SPTRSV_ERR_INVALID_ARG;
     return SPTRSV_OK;
   }
@@ -727,6 +733,18 @@ DevBuf &slot_buf<float>(sptrsv_plan_s *p) { return p->slots32; }
 template <typename T>
 using LaneKernel = void (*)(LaneArgs<T>);
 
+// Lane-engine kernel for R right-hand sides per lane (1, 2, 4 or 8).
+template <typename T>
+LaneKernel<T> pick_lane(bool bwd, int R) {
+  switch (R) {
+    case 1: return bwd ? k_exec_lane<T, true, 1> : k_exec_lane<T, false, 1>;
+    case 2: return bwd ? k_exec_lane<T, true, 2> : k_exec_lane<T, false, 2>;
+    case 4: return bwd ? k_exec_lane<T, true, 4> : k_exec_lane<T, false, 4>;
+    case 8: return bwd ? k_exec_lane<T, true, 8> : k_exec_lane<T, false, 8>;
+    default: return nullptr;
+  }
+}
+
 // Receiver wavefronts of a lane-engine workgroup: a quarter of them, at least
 // one, always leaving at least one compute wavefront.
 int lane_receivers(int block) {
@@ -753,20 +771,21 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
   ST_TRY(get_windows(p, c, &L->ws, s));
   if (c.engine == SPTRSV_ENGINE_LANE) {
     ST_TRY(get_lane_targets(p, c, L->ws, s));
-    L->R = 1;
-    L->ntiles = rhs;
-    L->lds = (size_t)L->ws->maxw * (sizeof(T) + sizeof(int));
+    L->R = c.rhs_per_lane;
+    L->ntiles = (rhs + L->R - 1) / L->R;
+    L->lds = (size_t)L->ws->maxw * ((size_t)L->R * sizeof(T) + sizeof(int));
     DevBuf &sb = slot_buf<T>(p);
     const size_t want =
-        std::max<size_t>(1, (size_t)L->ntiles * L->ws->nslots) * sizeof(T);
+        std::max<size_t>(1, (size_t)L->ntiles * L->ws->nslots * L->R) *
+        sizeof(T);
     if (want > sb.bytes) {
       ST_TRY(sb.ensure(want, false));
       ST_TRY(fill_async<T>(sb.as<T>(), sb.bytes / sizeof(T),
                            slot_empty_host<T>(), s));
     }
-    LaneKernel<T> kern = p->subst == SPTRSV_SUBSTITUTION_BACKWARD
-                             ? k_exec_lane<T, true>
-                             : k_exec_lane<T, false>;
+    LaneKernel<T> kern =
+        pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L->R);
+    if (!kern) return SPTRSV_ERR_INVALID_ARG;
     if (L->lds > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)kern,
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -840,9 +859,9 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.ticket = p->sync.as<int>();
     la.err = p->sync.as<int>() + 2;
     la.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
-    LaneKernel<T> kern = p->subst == SPTRSV_SUBSTITUTION_BACKWARD
-                             ? k_exec_lane<T, true>
-                             : k_exec_lane<T, false>;
+    LaneKernel<T> kern =
+        pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L.R);
+    if (!kern) return SPTRSV_ERR_INVALID_ARG;
     kern<<<L.grid, c.block, L.lds, s>>>(la);
     HIP_TRY(hipGetLastError());
     return SPTRSV_OK;
@@ -1070,9 +1089,17 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     c.partition = part;
     if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
   };
-  for (int block : {1024, 512, 256})
-    for (int window : {8192, 4096, 2048})
-      add(1, 1, 1, block, window, 0, SPTRSV_ENGINE_LANE);
+  for (int R : {1, 2, 4, 8}) {
+    if (R > 1 && R >= 2 * rhs) break;  // no tile narrower than half of R
+    int added = 0;  // the two widest windows that fit
+    for (int window : {8192, 4096, 2048, 1024}) {
+      if ((size_t)window * ((size_t)R * tsize + sizeof(int)) > p->lds_max)
+        continue;
+      for (int block : {1024, 512, 256})
+        add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
+      if (++added == 2) break;
+    }
+  }
   if (rhs <= 1) {
     for (int G : {1, 2, 4, 8, 16, 64})
       for (int block : {256, 1024}) {
@@ -1432,10 +1459,12 @@ int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg)
     if (c.window == 0) c.window = 4096;
   }
   if (c.lanes_per_column == 0) c.lanes_per_column = d.lanes_per_column;
-  // lanes over right-hand sides when the group is no wider than the default
-  // rhs lane count, otherwise lanes over nonzeros
+  // lanes over right-hand sides when the group is no wider than the smallest
+  // power of two covering rhs (at most 64), otherwise lanes over nonzeros
+  int rl = 1;
+  while (rl < rhs && rl < 64) rl <<= 1;
   if (c.rhs_lanes == 0)
-    c.rhs_lanes = d.rhs_lanes >= c.lanes_per_column ? c.lanes_per_column : 1;
+    c.rhs_lanes = rl >= c.lanes_per_column ? c.lanes_per_column : 1;
   if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
   if (c.block == 0) c.block = d.block;
   if (c.window == 0) c.window = c.block / c.lanes_per_column;
@@ -1755,13 +1784,24 @@ int dropin(const int *colPtr, const int *rowIdx, const T *val, int m, int n,
       printf("hip syncfree SpTRSV analysis on L used %4.2f ms\n",
              plan->analysis_ms);
     if (rhs > 1 && opt != SPTRSV_OPT_WARP_AUTO) {
+      // the reference's warp mappings (sptrsv_syncfree_cuda.h:290-407) on
+      // the group engine: a wavefront per column over its nonzeros, or
+      // lanes over the right-hand sides; AUTO keeps the default engine
       sptrsv_config_t c = default_config(plan, rhs);
+      c.engine = SPTRSV_ENGINE_GROUP;
+      c.block = 256;
       if (opt == SPTRSV_OPT_WARP_NNZ) {
         c.lanes_per_column = 64;
         c.rhs_lanes = 1;
         c.rhs_per_lane = rhs >= 4 ? 4 : 1;
-        c.block = 256;
         c.window = 4;
+      } else {
+        int rl = 1;
+        while (rl < rhs && rl < 64) rl <<= 1;
+        c.lanes_per_column = rl;
+        c.rhs_lanes = rl;
+        c.rhs_per_lane = rhs > 64 ? std::min(4, (rhs + 63) / 64) : 1;
+        c.window = std::max(1, 256 / rl);
       }
       if ((st = sptrsv_plan_set_config(plan, rhs, &c))) break;
     }

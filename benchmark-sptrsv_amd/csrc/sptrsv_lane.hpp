@@ -67,7 +67,7 @@ struct LaneArgs {
   const int4 *__restrict__ rlist; // columns with remote inputs, window order:
                                   // {local index, first slot, slot count, -}
   const int *__restrict__ roff;   // [nwin + 1] each window's rlist range
-  T *slots;                       // [ntiles][nslots]
+  T *slots;                       // [ntiles][nslots][R]
   const T *__restrict__ b;        // [n][rhs]
   T *__restrict__ x;              // [n][rhs]
   int n, rhs, nwin, ntiles, maxw, nslots;
@@ -101,30 +101,34 @@ __device__ __forceinline__ float dpp_t(float old, float v) {
 }
 
 // One step of an inclusive wavefront scan of affine maps x -> a + c * x_prev
-// with a readiness flag g: the element from the earlier lane (a1, c1, g1)
-// composes into this one as (a + c a1, c c1, g && (c == 0 || g1)).  The
-// identity (0, 1, 1) fills lanes without a source.
-template <typename T, int CTRL, int ROWMASK>
-__device__ __forceinline__ void scan_step(T &a, T &c, int &g) {
-  const T a1 = dpp_t<CTRL, ROWMASK>(T(0), a);
+// with a readiness flag g, for R right-hand sides sharing c and g: the
+// element from the earlier lane (a1, c1, g1) composes into this one as
+// (a + c a1, c c1, g && (c == 0 || g1)).  The identity (0, 1, 1) fills lanes
+// without a source.
+template <typename T, int R, int CTRL, int ROWMASK>
+__device__ __forceinline__ void scan_step(T (&a)[R], T &c, int &g) {
+  T a1[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) a1[q] = dpp_t<CTRL, ROWMASK>(T(0), a[q]);
   const T c1 = dpp_t<CTRL, ROWMASK>(T(1), c);
   const int g1 = dpp_i<CTRL, ROWMASK>(1, g);
   g = g & ((c == T(0) ? 1 : 0) | g1);
-  a = __builtin_fma(c, a1, a);
+#pragma unroll
+  for (int q = 0; q < R; q++) a[q] = __builtin_fma(c, a1[q], a[q]);
   c = c * c1;
 }
 
 // Inclusive scan over the 64 lanes: row_shr 1, 2, 4, 8 inside each row of
 // 16, then row_bcast:15 into rows 1 and 3, then row_bcast:31 into rows 2
 // and 3 (the gfx9 DPP controls 0x111..0x118, 0x142, 0x143).
-template <typename T>
-__device__ __forceinline__ void chain_scan(T &a, T &c, int &g) {
-  scan_step<T, 0x111, 0xf>(a, c, g);
-  scan_step<T, 0x112, 0xf>(a, c, g);
-  scan_step<T, 0x114, 0xf>(a, c, g);
-  scan_step<T, 0x118, 0xf>(a, c, g);
-  scan_step<T, 0x142, 0xa>(a, c, g);
-  scan_step<T, 0x143, 0xc>(a, c, g);
+template <typename T, int R>
+__device__ __forceinline__ void chain_scan(T (&a)[R], T &c, int &g) {
+  scan_step<T, R, 0x111, 0xf>(a, c, g);
+  scan_step<T, R, 0x112, 0xf>(a, c, g);
+  scan_step<T, R, 0x114, 0xf>(a, c, g);
+  scan_step<T, R, 0x118, 0xf>(a, c, g);
+  scan_step<T, R, 0x142, 0xa>(a, c, g);
+  scan_step<T, R, 0x143, 0xc>(a, c, g);
 }
 
 // An entry delivered inside its wavefront instead of through LDS or a slot
@@ -133,27 +137,30 @@ __device__ __forceinline__ void chain_scan(T &a, T &c, int &g) {
 constexpr int kRegTarget = -2147483647 - 1;
 
 // Compute wavefront cw handles batches of 64 consecutive columns (lane =
-// column) cw, cw + ncw, ...  Each row r may have one in-batch predecessor
-// p(r) whose entry is delivered by register (analysis: `chain`); every other
-// input arrives through LDS (same window) or a slot (earlier window, folded
-// into LDS by the receivers).  Once a lane's LDS arrivals are complete its
-// value is affine in its predecessor's:
+// column) cw, cw + ncw, ..., each column for the R right-hand sides of its
+// tile (rv of them valid).  Each row r may have one in-batch predecessor p(r)
+// whose entry is delivered by register (analysis: `chain`); every other input
+// arrives through LDS (same window) or a slot (earlier window, folded into
+// LDS by the receivers).  Once a lane's LDS arrivals are complete its value
+// is affine in its predecessor's:
 //     x_r = r_r / d_r + c_r * x_p(r),   c_r = -L(r, p(r)) / d_r,  |c_r| <= 1,
 // so the lanes whose inputs are complete along their whole predecessor chain
-// are solved together by pointer jumping over the batch's forest
-// (log2(chain depth) rounds of cross-lane permutes), instead of one lane per
-// step.  Solved lanes publish at once: LDS partial values, then (after a
-// workgroup release) LDS arrival counters; slots are written first, since a
-// slot's value is its own arrival flag.  Lanes whose chain is not complete
-// wait for the next round; the lowest unsolved lane always completes once its
-// inputs from earlier batches arrive, so the batch always drains.
-template <typename T, bool BWD>
+// are solved together: by a DPP scan when every link is lane -> lane + 1,
+// otherwise by pointer jumping over the batch's forest (log2(chain depth)
+// rounds of cross-lane permutes), instead of one lane per step.  Solved lanes
+// publish at once: slots first (a slot's value is its own arrival flag), then
+// LDS partial values, a workgroup release, then LDS arrival counters.  Lanes
+// whose chain is not complete wait for the next round; the lowest unsolved
+// lane always completes once its inputs from earlier batches arrive, so the
+// batch always drains.
+template <typename T, bool BWD, int R>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int wlo, int wlen,
-                                            int tile, int cw, int ncw) {
+                                            int tile, int rv, int cw,
+                                            int ncw) {
   const int lane = __lane_id();
   const int n = a.n;
-  T *slots = a.slots + (size_t)tile * a.nslots;
+  T *slots = a.slots + (size_t)tile * a.nslots * R;
   bool stopped = false;  // the stall watch fired: leave the window
   for (int base = cw * kWave; base < wlen && !stopped; base += ncw * kWave) {
     const int k = base + lane;
@@ -173,7 +180,8 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       ne = min(hi - lo, kPF);
       const int2 nd = a.need[c];
       const int2 ch = a.chain[c];
-      need_lds = nd.x + nd.y;
+      // receivers count one arrival per remote entry and right-hand side
+      need_lds = nd.x + nd.y * rv;
       if (ch.x >= 0 && ch.x < lane) {
         pl = ch.x;
         need_lds -= 1;
@@ -193,27 +201,37 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     // DPP instead of pointer jumping through LDS permutes
     const bool chain_batch =
         __ballot(active && pl != lane && pl != lane - 1) == 0;
-    T xs = T(0);  // x of the lanes solved in earlier rounds
+    T xs[R];  // x of the lanes solved in earlier rounds
+#pragma unroll
+    for (int q = 0; q < R; q++) xs[q] = T(0);
+    // ready-lane mask of the last round that solved nothing: until it
+    // changes no lane can become solvable, so the round is not repeated
+    unsigned long long stale = 0;
     StallWatch watch;
     int level = 0;
     while (__ballot(active) != 0) {
       const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
       const bool ext = active && cnt == need_lds;
       bool progressed = false;  // some lane solved in this round
-      if (__ballot(ext) != 0) {
+      const unsigned long long extm = __ballot(ext);
+      if (extm != 0 && extm != stale) {
         fence_acq_wg();
-        const T r = ext ? s_val[k] : T(0);
-        T q = r * inv;
-        q = __builtin_fma(__builtin_fma(-q, diag, r), inv, q);
-        // (av, cf, pp): x = av + cf * x_pp; solved lanes are roots (cf = 0)
-        T av = active ? q : xs;
+        // (av, cf): x = av + cf * x_pred; solved lanes are roots (cf = 0)
+        T av[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+          const T r = ext ? s_val[k * R + q] : T(0);
+          T qv = r * inv;
+          qv = __builtin_fma(__builtin_fma(-qv, diag, r), inv, qv);
+          av[q] = active ? qv : xs[q];
+        }
         T cf = active ? coef : T(0);
         bool ok = active ? ext : true;
         if (chain_batch) {
           // lane 0 never has an in-batch predecessor, so after the full
           // scan every cf is 0 and av is x wherever g holds
           int g = ok ? 1 : 0;
-          chain_scan<T>(av, cf, g);
+          chain_scan<T, R>(av, cf, g);
           ok = g != 0;
         } else {
           int pp = active ? pl : lane;
@@ -224,9 +242,12 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           // are below the smallest normal number times the ancestors' values.
           while (__ballot(ok && cf != T(0)) != 0) {
             const int qk = __shfl(pp | (ok ? 0 : 64), pp);
-            const T ap = __shfl(av, pp);
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+              const T ap = __shfl(av[q], pp);
+              av[q] = __builtin_fma(cf, ap, av[q]);
+            }
             const T cp = __shfl(cf, pp);
-            av = __builtin_fma(cf, ap, av);
             cf = cf * cp;
             ok = ok && (qk & 64) == 0;
             pp = qk & 63;
@@ -234,28 +255,45 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         }
         const bool solved = active && ok;
         if (solved) {
-          xs = av;
+#pragma unroll
+          for (int q = 0; q < R; q++) xs[q] = av[q];
           active = false;
         }
         progressed = __ballot(solved) != 0;
+        stale = progressed ? 0 : extm;
         if (progressed) {
-          const T xv = av;
           if (solved) {
             // slots first: the value is its own arrival flag
 #pragma unroll
             for (int e = 0; e < kPF; e++)
-              if (e < ne && tg[e] >= 0) xchg_agent(&slots[tg[e]], xv * vv[e]);
-            s_val[k] = xv;  // the column is final: its LDS value now holds x
+              if (e < ne && tg[e] >= 0) {
+#pragma unroll
+                for (int q = 0; q < R; q++)
+                  if (q < rv)
+                    xchg_agent(&slots[(size_t)tg[e] * R + q], av[q] * vv[e]);
+              }
+#pragma unroll
+            for (int q = 0; q < R; q++)  // the column is final: LDS holds x
+              s_val[k * R + q] = av[q];
 #pragma unroll
             for (int e = 0; e < kPF; e++)
-              if (e < ne && tg[e] < 0 && tg[e] != kRegTarget)
-                add_wg(&s_val[-tg[e] - 1], -(xv * vv[e]));
+              if (e < ne && tg[e] < 0 && tg[e] != kRegTarget) {
+#pragma unroll
+                for (int q = 0; q < R; q++)
+                  if (q < rv) add_wg(&s_val[(-tg[e] - 1) * R + q], -(av[q] * vv[e]));
+              }
             for (int j = lo + kPF; j < hi; j++) {
               const int t = a.etgt[j];
-              if (t >= 0)
-                xchg_agent(&slots[t], xv * a.val[j]);
-              else if (t != kRegTarget)
-                add_wg(&s_val[-t - 1], -(xv * a.val[j]));
+              const T v = a.val[j];
+              if (t >= 0) {
+#pragma unroll
+                for (int q = 0; q < R; q++)
+                  if (q < rv) xchg_agent(&slots[(size_t)t * R + q], av[q] * v);
+              } else if (t != kRegTarget) {
+#pragma unroll
+                for (int q = 0; q < R; q++)
+                  if (q < rv) add_wg(&s_val[(-t - 1) * R + q], -(av[q] * v));
+              }
             }
           }
           fence_rel_wg();
@@ -287,37 +325,43 @@ constexpr int kTake = 8;  // slots of one row taken per group
 
 // Receivers: the first nrecv wavefronts walk the window's receive list (only
 // the columns that have inputs from earlier windows, precomputed per window
-// partition), lane l of receiver r taking entries r*64 + l, stride nrecv*64.
-// A column's slots are taken in groups of up to kTake with
-// exchange(slot, EMPTY), which both polls and resets a slot.  Only the first
-// slot of a group is polled until it arrives; the rest of the group is taken
-// after it, and the group is summed in slot order once complete, so the
-// remote sum has a fixed order.  A receiver wavefront that took nothing in an
-// iteration backs off.
-template <typename T>
+// partition), one (column, right-hand side) pair per lane: lane l of
+// receiver r takes pairs r*64 + l, stride nrecv*64.  A pair's slots are taken
+// in groups of up to kTake with exchange(slot, EMPTY), which both polls and
+// resets a slot.  Only the first slot of a group is polled until it arrives;
+// the rest of the group is taken after it, and the group is summed in slot
+// order once complete, so the remote sum has a fixed order.  A receiver
+// wavefront that took nothing in an iteration backs off.
+template <typename T, int R>
 __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int win, int tile,
-                                            int rw) {
-  T *slots = a.slots + (size_t)tile * a.nslots;
+                                            int rv, int rw) {
+  T *slots = a.slots + (size_t)tile * a.nslots * R;
   const T empty = slot_empty_value<T>();
-  const int e_end = a.roff[win + 1];
+  const int e0 = a.roff[win];
+  const int p_end = (a.roff[win + 1] - e0) * rv;
   const int stride = a.nrecv * kWave;
-  int e = a.roff[win] + rw * kWave + __lane_id();
-  int k = 0, s = 0, r1 = 0, m = 0;
+  int pi = rw * kWave + __lane_id();
+  int k = 0, s = 0, r1 = 0, m = 0, q = 0;
   unsigned got = 0;
   T part[kTake];
   T sum = T(0);
-  int4 next = e < e_end ? a.rlist[e] : make_int4(0, 0, 0, 0);
+  auto fetch = [&](int i) {
+    const int e = R == 1 ? i : i / rv;
+    return a.rlist[e0 + e];
+  };
+  int4 next = pi < p_end ? fetch(pi) : make_int4(0, 0, 0, 0);
   auto take_next = [&]() {
     k = next.x;
     s = next.y;
     m = next.z;
+    q = R == 1 ? 0 : pi % rv;
     r1 = s + m;
     got = 0;
     sum = T(0);
-    if (e + stride < e_end) next = a.rlist[e + stride];  // prefetch
+    if (pi + stride < p_end) next = fetch(pi + stride);  // prefetch
   };
-  bool active = e < e_end;
+  bool active = pi < p_end;
   if (active) take_next();
   StallWatch watch;
   int level = 0;
@@ -328,25 +372,25 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
       const int g = min(kTake, r1 - s);
       const unsigned before = got;
 #pragma unroll
-      for (int q = 0; q < kTake; q++) {
+      for (int i = 0; i < kTake; i++) {
         // slot 0 of the group first; the others once slot 0 is in
-        if (q < g && !(got & (1u << q)) && (q == 0 || (got & 1u))) {
-          const T v = xchg_take(&slots[s + q], empty);
+        if (i < g && !(got & (1u << i)) && (i == 0 || (got & 1u))) {
+          const T v = xchg_take(&slots[(size_t)(s + i) * R + q], empty);
           if (!slot_empty(v)) {
-            part[q] = v;
-            got |= 1u << q;
+            part[i] = v;
+            got |= 1u << i;
           }
         }
       }
       progressed = got != before;
       if (got == (1u << g) - 1) {
 #pragma unroll
-        for (int q = 0; q < kTake; q++)
-          if (q < g) sum += part[q];
+        for (int i = 0; i < kTake; i++)
+          if (i < g) sum += part[i];
         s += g;
         got = 0;
         if (s == r1) {
-          add_wg(&s_val[k], -sum);
+          add_wg(&s_val[k * R + q], -sum);
           fence_rel_wg();
           add_wg(&s_cnt[k], m);
           done = true;
@@ -354,8 +398,8 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
       }
     }
     if (done) {
-      e += stride;
-      active = e < e_end;
+      pi += stride;
+      active = pi < p_end;
       if (active) take_next();
     }
     const bool any = __ballot(progressed) != 0;
@@ -364,11 +408,11 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   }
 }
 
-template <typename T, bool BWD>
+template <typename T, bool BWD, int R>
 __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
-  int *s_cnt = reinterpret_cast<int *>(s_val + a.maxw);
+  int *s_cnt = reinterpret_cast<int *>(s_val + (size_t)a.maxw * R);
   __shared__ int s_ticket;
   const int total = a.nwin * a.ntiles;
   const int wave = threadIdx.x / kWave;
@@ -383,25 +427,30 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
     if (t >= total) break;
     const int win = t / a.ntiles;
     const int tile = t - win * a.ntiles;
+    const int rv = min(R, a.rhs - tile * R);  // valid right-hand sides
     const int wlo = a.winPtr[win];
     const int wlen = a.winPtr[win + 1] - wlo;
-    for (int k = threadIdx.x; k < wlen; k += blockDim.x) {
+    for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
+      const int k = R == 1 ? i : i / R;
+      const int q = R == 1 ? 0 : i - k * R;
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
-      s_val[k] = a.b[(size_t)c * a.rhs + tile];
-      s_cnt[k] = 0;
+      s_val[i] = q < rv ? a.b[(size_t)c * a.rhs + tile * R + q] : T(0);
     }
+    for (int k = threadIdx.x; k < wlen; k += blockDim.x) s_cnt[k] = 0;
     __syncthreads();
     if (wave < a.nrecv)
-      lane_receive<T>(a, s_val, s_cnt, win, tile, wave);
+      lane_receive<T, R>(a, s_val, s_cnt, win, tile, rv, wave);
     else
-      lane_compute<T, BWD>(a, s_val, s_cnt, wlo, wlen, tile, wave - a.nrecv,
-                           ncw);
+      lane_compute<T, BWD, R>(a, s_val, s_cnt, wlo, wlen, tile, rv,
+                              wave - a.nrecv, ncw);
     __syncthreads();
-    for (int k = threadIdx.x; k < wlen; k += blockDim.x) {
+    for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
+      const int k = R == 1 ? i : i / R;
+      const int q = R == 1 ? 0 : i - k * R;
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
-      a.x[(size_t)c * a.rhs + tile] = s_val[k];
+      if (q < rv) a.x[(size_t)c * a.rhs + tile * R + q] = s_val[i];
     }
   }
   if (threadIdx.x == 0) {

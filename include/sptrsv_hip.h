@@ -181,9 +181,9 @@ typedef struct sptrsv_plan_s *sptrsv_plan_t;
  * consecutive columns whose partial sums and counters live in LDS.
  * A zero field means "choose". */
 typedef struct {
-  int lanes_per_column; /* 1,2,4,...,64 */
-  int rhs_lanes;        /* 1 or lanes_per_column */
-  int rhs_per_lane;     /* 1..4 */
+  int lanes_per_column; /* 1,2,4,...,64 (lane engine: 1) */
+  int rhs_lanes;        /* 1 or lanes_per_column (lane engine: 1) */
+  int rhs_per_lane;     /* group engine 1..4; lane engine 1, 2, 4 or 8 */
   int block;            /* threads per workgroup: 64..1024, multiple of 64 */
   int window;           /* LDS window, columns (max, for NNZ partition) */
   int partition;        /* SPTRSV_COL_PARTITION / SPTRSV_NNZ_PARTITION */
@@ -193,10 +193,13 @@ typedef struct {
 
 /* Executor engines.  GROUP: lane groups of a wavefront per column, counters
  * and partial sums in LDS within a window and in HBM (atomics + fences)
- * across windows -- all rhs modes.  LANE: one lane per column, compute
- * wavefronts that touch only LDS, and one value-carrying slot per
- * cross-window entry polled by a receiver wavefront -- one right-hand side per
- * tile (lanes_per_column = rhs_lanes = rhs_per_lane = 1, block >= 128). */
+ * across windows -- all rhs modes.  LANE (the default): one lane per column
+ * carrying rhs_per_lane right-hand sides of a tile, chains inside a
+ * 64-column batch solved by a wavefront scan, compute wavefronts that touch
+ * only LDS, and one value-carrying slot per cross-window entry and
+ * right-hand side polled by receiver wavefronts (lanes_per_column =
+ * rhs_lanes = 1, block >= 128; LDS = window * (rhs_per_lane * value size + 4)
+ * bytes). */
 #define SPTRSV_ENGINE_GROUP 0
 #define SPTRSV_ENGINE_LANE 1
 

@@ -212,14 +212,17 @@ RHS_CONFIGS = [None,
                dict(lanes_per_column=64, rhs_lanes=1, rhs_per_lane=4, block=256, window=4),
                dict(lanes_per_column=1, rhs_lanes=1, rhs_per_lane=1, block=256, window=256),
                dict(lanes_per_column=4, rhs_lanes=1, rhs_per_lane=4, block=256, window=64),
-               dict(engine=S.ENGINE_LANE, block=256, window=512)]
+               dict(engine=S.ENGINE_LANE, block=256, window=512),
+               dict(engine=S.ENGINE_LANE, rhs_per_lane=2, block=512, window=1024),
+               dict(engine=S.ENGINE_LANE, rhs_per_lane=4, block=1024, window=2048),
+               dict(engine=S.ENGINE_LANE, rhs_per_lane=8, block=256, window=300)]
 
 
 def _rhs_id(c):
     if c is None:
         return "default"
     if c.get("engine") == S.ENGINE_LANE:
-        return "lane-B{block}-W{window}".format(**c)
+        return "lane-R{r}-B{block}-W{window}".format(r=c.get("rhs_per_lane", 1), **c)
     return "G{lanes_per_column}-RL{rhs_lanes}-r{rhs_per_lane}".format(**c)
 
 
@@ -231,6 +234,17 @@ def test_multi_rhs_matches_oracle(rhs, cfg):
     B = f.spmv(X)
     x, _ = plan_solve(f, B, rhs=rhs, cfg=cfg)
     assert_close(x, oracle_x(f, B, rhs=rhs), 1e-12)
+
+
+@pytest.mark.parametrize("R", [2, 4, 8])
+def test_multi_rhs_lane_forest_backward(R):
+    # power-law batches are not chains: pointer jumping with R values per lane
+    f = mx.upper_from_lower(mx.power_law_lower(6000, window=700, seed=4))
+    X = mx.x_normal(f.n, 7, seed=5)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, rhs=7, subst=1,
+                      cfg=dict(engine=S.ENGINE_LANE, rhs_per_lane=R, block=512, window=1000))
+    assert_close(x, oracle_x(f, B, rhs=7, subst=1), 1e-12)
 
 
 def test_multi_rhs_integer_bit_exact_backward():
