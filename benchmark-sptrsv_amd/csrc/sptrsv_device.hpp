@@ -99,21 +99,6 @@ __device__ __forceinline__ void backoff(int &level, bool progressed) {
   level = level == 0 ? 1 : min(level * 2, kBackoffMax);
 }
 
-// Back-off for wavefronts polling LDS: at most 2 * 64 cycles, so a result
-// produced inside the workgroup is still picked up within a few hundred
-// cycles.
-__device__ __forceinline__ void backoff_short(int &level, bool progressed) {
-  if (progressed) {
-    level = 0;
-    return;
-  }
-  if (level >= 2)
-    __builtin_amdgcn_s_sleep(2);
-  else if (level == 1)
-    __builtin_amdgcn_s_sleep(1);
-  level = min(level + 1, 2);
-}
-
 // Per-wavefront stall watch for spin-waits.  Only lane 0 touches the clock
 // and the shared error word, and only every 1024 stalled iterations, so the
 // watch adds no traffic to the coherence point (one shared word polled by

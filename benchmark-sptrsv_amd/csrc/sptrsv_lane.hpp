@@ -208,13 +208,14 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     // changes no lane can become solvable, so the round is not repeated
     unsigned long long stale = 0;
     StallWatch watch;
-    int level = 0;
     while (__ballot(active) != 0) {
       const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
       const bool ext = active && cnt == need_lds;
       bool progressed = false;  // some lane solved in this round
       const unsigned long long extm = __ballot(ext);
       if (extm != 0 && extm != stale) {
+        // a solving wavefront takes issue priority over the SIMD's pollers
+        __builtin_amdgcn_s_setprio(3);
         fence_acq_wg();
         // (av, cf): x = av + cf * x_pred; solved lanes are roots (cf = 0)
         T av[R];
@@ -309,10 +310,11 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           }
         }
       }
-      // nothing solved: short back-off, so idle wavefronts leave LDS
-      // bandwidth to the ones that are solving; a wavefront whose lanes have
-      // inputs but wait on an unsolved predecessor counts as stalled
-      backoff_short(level, progressed);
+      __builtin_amdgcn_s_setprio(0);
+      // Pollers spin on LDS without sleeping: the priority above keeps them
+      // from delaying a solving wavefront, and a sleep would only delay the
+      // poll that sees the next arrival.  A wavefront whose lanes have inputs
+      // but wait on an unsolved predecessor counts as stalled.
       if (watch.step(progressed, a.err, a.wait_cap)) {
         active = false;
         stopped = true;
