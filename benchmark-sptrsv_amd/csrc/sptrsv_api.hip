@@ -419,7 +419,7 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   c.block = 1024;
   c.window = 8192;
   const size_t per_col = (size_t)c.rhs_per_lane * 8 + sizeof(int);
-  while (c.window > 256 && (size_t)c.window * per_col > p->lds_max)
+  while (c.window > 256 && ((size_t)c.window + 64) * per_col > p->lds_max)
     c.window /= 2;
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
@@ -438,7 +438,8 @@ int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
     if (c.partition != SPTRSV_COL_PARTITION &&
         c.partition != SPTRSV_NNZ_PARTITION)
       return SPTRSV_ERR_INVALID_ARG;
-    if ((size_t)c.window * ((size_t)R * tsize + sizeof(int)) > p->lds_max)
+    if (((size_t)c.window + 64) * ((size_t)R * tsize + sizeof(int)) >
+        p->lds_max)
       return SPTRSV_ERR_INVALID_ARG;
     return SPTRSV_OK;
   }
@@ -773,7 +774,8 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     ST_TRY(get_lane_targets(p, c, L->ws, s));
     L->R = c.rhs_per_lane;
     L->ntiles = (rhs + L->R - 1) / L->R;
-    L->lds = (size_t)L->ws->maxw * ((size_t)L->R * sizeof(T) + sizeof(int));
+    // window rows plus one dummy row per lane
+    L->lds = ((size_t)L->ws->maxw + 64) * ((size_t)L->R * sizeof(T) + sizeof(int));
     DevBuf &sb = slot_buf<T>(p);
     const size_t want =
         std::max<size_t>(1, (size_t)L->ntiles * L->ws->nslots * L->R) *
@@ -1093,7 +1095,8 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     if (R > 1 && R >= 2 * rhs) break;  // no tile narrower than half of R
     int added = 0;  // the two widest windows that fit
     for (int window : {8192, 4096, 2048, 1024}) {
-      if ((size_t)window * ((size_t)R * tsize + sizeof(int)) > p->lds_max)
+      if (((size_t)window + 64) * ((size_t)R * tsize + sizeof(int)) >
+          p->lds_max)
         continue;
       for (int block : {1024, 512, 256})
         add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);

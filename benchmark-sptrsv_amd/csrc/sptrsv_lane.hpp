@@ -101,18 +101,15 @@ __device__ __forceinline__ float dpp_t(float old, float v) {
 }
 
 // One step of an inclusive wavefront scan of affine maps x -> a + c * x_prev
-// with a readiness flag g, for R right-hand sides sharing c and g: the
-// element from the earlier lane (a1, c1, g1) composes into this one as
-// (a + c a1, c c1, g && (c == 0 || g1)).  The identity (0, 1, 1) fills lanes
-// without a source.
+// for R right-hand sides sharing c: the element from the earlier lane
+// (a1, c1) composes into this one as (a + c a1, c c1).  The identity (0, 1)
+// fills lanes without a source.
 template <typename T, int R, int CTRL, int ROWMASK>
-__device__ __forceinline__ void scan_step(T (&a)[R], T &c, int &g) {
+__device__ __forceinline__ void scan_step(T (&a)[R], T &c) {
   T a1[R];
 #pragma unroll
   for (int q = 0; q < R; q++) a1[q] = dpp_t<CTRL, ROWMASK>(T(0), a[q]);
   const T c1 = dpp_t<CTRL, ROWMASK>(T(1), c);
-  const int g1 = dpp_i<CTRL, ROWMASK>(1, g);
-  g = g & ((c == T(0) ? 1 : 0) | g1);
 #pragma unroll
   for (int q = 0; q < R; q++) a[q] = __builtin_fma(c, a1[q], a[q]);
   c = c * c1;
@@ -122,13 +119,28 @@ __device__ __forceinline__ void scan_step(T (&a)[R], T &c, int &g) {
 // 16, then row_bcast:15 into rows 1 and 3, then row_bcast:31 into rows 2
 // and 3 (the gfx9 DPP controls 0x111..0x118, 0x142, 0x143).
 template <typename T, int R>
-__device__ __forceinline__ void chain_scan(T (&a)[R], T &c, int &g) {
-  scan_step<T, R, 0x111, 0xf>(a, c, g);
-  scan_step<T, R, 0x112, 0xf>(a, c, g);
-  scan_step<T, R, 0x114, 0xf>(a, c, g);
-  scan_step<T, R, 0x118, 0xf>(a, c, g);
-  scan_step<T, R, 0x142, 0xa>(a, c, g);
-  scan_step<T, R, 0x143, 0xc>(a, c, g);
+__device__ __forceinline__ void chain_scan(T (&a)[R], T &c) {
+  scan_step<T, R, 0x111, 0xf>(a, c);
+  scan_step<T, R, 0x112, 0xf>(a, c);
+  scan_step<T, R, 0x114, 0xf>(a, c);
+  scan_step<T, R, 0x118, 0xf>(a, c);
+  scan_step<T, R, 0x142, 0xa>(a, c);
+  scan_step<T, R, 0x143, 0xc>(a, c);
+}
+
+// Readiness along lane -> lane + 1 links: lane i can be solved when no lane
+// of its segment (from the nearest lane at or below i that has no link, up
+// to i) is unready.  linked: bit i set when lane i takes lane i - 1's value;
+// ready: bit i set when lane i has its inputs (or is already solved).
+__device__ __forceinline__ bool chain_ready(unsigned long long linked,
+                                            unsigned long long ready,
+                                            int lane) {
+  const unsigned long long upto =
+      lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+  const unsigned long long roots = ~linked & upto;  // bit 0 is always set
+  const int s = 63 - __clzll((long long)roots);
+  const unsigned long long seg = upto & ~((1ull << s) - 1);
+  return (~ready & seg) == 0;
 }
 
 // An entry delivered inside its wavefront instead of through LDS or a slot
@@ -197,6 +209,19 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         vv[i] = j < hi ? a.val[j] : T(0);
       }
     }
+    // LDS target of each register entry; entries that do not go to LDS add
+    // 0 to this lane's own dummy row (past the window), so the publish is
+    // straight-line code with no per-entry branches
+    int ladr[kPF];
+    T lval[kPF];
+    int linc[kPF];
+#pragma unroll
+    for (int i = 0; i < kPF; i++) {
+      const bool loc = i < ne && tg[i] < 0 && tg[i] != kRegTarget;
+      ladr[i] = loc ? -tg[i] - 1 : a.maxw + lane;
+      lval[i] = loc ? vv[i] : T(0);
+      linc[i] = loc ? 1 : 0;
+    }
     // every predecessor link is lane - 1 (stencil rows, chains): scan with
     // DPP instead of pointer jumping through LDS permutes
     const bool chain_batch =
@@ -230,10 +255,11 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         bool ok = active ? ext : true;
         if (chain_batch) {
           // lane 0 never has an in-batch predecessor, so after the full
-          // scan every cf is 0 and av is x wherever g holds
-          int g = ok ? 1 : 0;
-          chain_scan<T, R>(av, cf, g);
-          ok = g != 0;
+          // scan every cf is 0 and av is x wherever the segment is ready
+          const unsigned long long linked = __ballot(cf != T(0));
+          const unsigned long long ready = __ballot(ok);
+          chain_scan<T, R>(av, cf);
+          ok = chain_ready(linked, ready, lane);
         } else {
           int pp = active ? pl : lane;
           // Every lane runs the rounds, so every permute reads a live lane.
@@ -276,13 +302,12 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 #pragma unroll
             for (int q = 0; q < R; q++)  // the column is final: LDS holds x
               s_val[k * R + q] = av[q];
+            // right-hand sides past rv hold x = 0 and add 0
 #pragma unroll
             for (int e = 0; e < kPF; e++)
-              if (e < ne && tg[e] < 0 && tg[e] != kRegTarget) {
 #pragma unroll
-                for (int q = 0; q < R; q++)
-                  if (q < rv) add_wg(&s_val[(-tg[e] - 1) * R + q], -(av[q] * vv[e]));
-              }
+              for (int q = 0; q < R; q++)
+                add_wg(&s_val[ladr[e] * R + q], -(av[q] * lval[e]));
             for (int j = lo + kPF; j < hi; j++) {
               const int t = a.etgt[j];
               const T v = a.val[j];
@@ -300,9 +325,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           fence_rel_wg();
           if (solved) {
 #pragma unroll
-            for (int e = 0; e < kPF; e++)
-              if (e < ne && tg[e] < 0 && tg[e] != kRegTarget)
-                add_wg(&s_cnt[-tg[e] - 1], 1);
+            for (int e = 0; e < kPF; e++) add_wg(&s_cnt[ladr[e]], linc[e]);
             for (int j = lo + kPF; j < hi; j++) {
               const int t = a.etgt[j];
               if (t < 0 && t != kRegTarget) add_wg(&s_cnt[-t - 1], 1);
@@ -414,7 +437,9 @@ template <typename T, bool BWD, int R>
 __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
-  int *s_cnt = reinterpret_cast<int *>(s_val + (size_t)a.maxw * R);
+  // maxw window rows plus 64 dummy rows (one per lane) for branch-free
+  // publishing
+  int *s_cnt = reinterpret_cast<int *>(s_val + (size_t)(a.maxw + kWave) * R);
   __shared__ int s_ticket;
   const int total = a.nwin * a.ntiles;
   const int wave = threadIdx.x / kWave;
