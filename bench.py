@@ -54,6 +54,13 @@ def make_workload(name):
     if name == "batch":
         f = mx.laplacian2d_batch(512, seed=0)
         return f, mx.x_uniform(f.n, seed=0), None
+    if name == "lap3d_share":
+        # one GPU's share of the 3D Laplacian under an 8-way contiguous column
+        # split: the diagonal block of 16 planes.  Its right-hand side is
+        # L_block x_block, i.e. the contributions of the earlier blocks are
+        # already subtracted, as if the remote producers had finished.
+        f = mx.laplacian3d_lower(128, 128, 16)
+        return f, mx.x_uniform(f.n, seed=0), 127 + 127 + 15 + 1
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -63,10 +70,45 @@ WORKLOAD_DESC = {
     "powerlaw": "random power-law lower triangular n=1,000,000, Zipf(2) cap 1000, window 50,000",
     "bidiag": "bidiagonal chain n=1,000,000, diag 2, sub -1",
     "batch": "512 uneven 2D 5-point Laplacian lower factors (sides log-U[32,256]) in one launch",
+    "lap3d_share": "one GPU's share of the 128^3 3D Laplacian factor under an 8-way "
+                   "contiguous column split (diagonal block of 16 planes, n=262,144; "
+                   "cross-block contributions pre-accumulated into b)",
 }
 
 
-def cpu_baseline(f, b, budget_s):
+def per_system_launches(f, b, repeats=3):
+    """The batch's systems solved the plain way: one plan and one launch per
+    system, back to back on one stream (default executor configuration, since
+    tuning 512 plans would dominate the run).  Returns the mean wall time of
+    the 512 launches in ms, the solution arrays and each system's span."""
+    import sptrsv_amd as S
+    sp = f.sys_ptr.astype(np.int64)
+    plans = []
+    for s in range(len(sp) - 1):
+        lo, hi = int(sp[s]), int(sp[s + 1])
+        e0, e1 = int(f.colPtr[lo]), int(f.colPtr[hi])
+        cp = (f.colPtr[lo:hi + 1] - e0).astype(np.int32)
+        ri = (f.rowIdx[e0:e1] - lo).astype(np.int32)
+        plan = S.Plan.from_host(cp, ri, f.val[e0:e1])
+        db = S.DeviceArray.from_host(np.ascontiguousarray(b[lo:hi]))
+        dx = S.DeviceArray((hi - lo,), np.float64)
+        plans.append((plan, db, dx, lo, hi))
+    stream = S.Stream()
+    for plan, db, dx, _, _ in plans:  # warm-up: analysis caches, slots
+        plan.solve(db, dx, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        for plan, db, dx, _, _ in plans:
+            plan.solve(db, dx, stream=stream)
+    stream.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / repeats
+    for plan, _, _, _, _ in plans:
+        assert plan.check(stream) == 0, "per-system solve timed out"
+    return ms, [p[2] for p in plans], [(p[3], p[4]) for p in plans]
+
+
+def cpu_baseline(f, b, budget_s, substitution=0):
     """Single-threaded serial sync-free executor on the same matrix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import sptrsv_oracle as orc
@@ -78,7 +120,7 @@ def cpu_baseline(f, b, budget_s):
     times = []
     while time.perf_counter() - t0 < budget_s or reps < 2:
         t = time.perf_counter()
-        x_cpu = fn(colPtr, rowIdx, f.val, b, in_degree=indeg)
+        x_cpu = fn(colPtr, rowIdx, f.val, b, substitution=substitution, in_degree=indeg)
         times.append(time.perf_counter() - t)
         reps += 1
     med = float(np.median(times))
@@ -94,6 +136,11 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="lap3d", choices=sorted(WORKLOAD_DESC))
+    ap.add_argument("--substitution", default="forward", choices=["forward", "backward"],
+                    help="backward solves U = L^T of the workload (diagonal last)")
+    ap.add_argument("--mtx", default=None,
+                    help="Matrix Market file: its unit-diagonal triangle (main.cu:152-187)")
+    ap.add_argument("--csc", default=None, help="CSC text factor (main.cu:247-286)")
     ap.add_argument("--rhs", type=int, default=1)
     ap.add_argument("--form", default="plain", choices=["plain", "ir"])
     ap.add_argument("--ir-iters", type=int, default=2)
@@ -129,7 +176,20 @@ def main():
         raise SystemExit("no HIP device visible")
     log(f"device: {S.device_name()}")
     t = time.perf_counter()
-    f, x_true, levels = make_workload(args.workload)
+    bwd = args.substitution == "backward"
+    if args.mtx or args.csc:
+        from sptrsv_amd import matrices as mx, mtx
+        f = (mtx.triangle_from_mtx(args.mtx, substitution=int(bwd)) if args.mtx
+             else mtx.read_csc_text(args.csc))
+        x_true, levels = mx.x_uniform(f.n, seed=0), None
+        args.workload = "file"
+        WORKLOAD_DESC["file"] = f"{'Matrix Market' if args.mtx else 'CSC text'} factor " \
+                                f"{os.path.basename(args.mtx or args.csc)} (n={f.n}, nnz={f.nnz})"
+    else:
+        f, x_true, levels = make_workload(args.workload)
+        if bwd:
+            from sptrsv_amd import matrices as mx
+            f = mx.upper_from_lower(f)
     rhs = args.rhs
     if rhs > 1:
         from sptrsv_amd import matrices as mx
@@ -142,7 +202,8 @@ def main():
     dP = S.DeviceArray.from_host(f.colPtr)
     dI = S.DeviceArray.from_host(f.rowIdx)
     dV = S.DeviceArray.from_host(f.val)
-    plan = S.Plan(dP, dI, dV, f.n, f.nnz, flags=flags, sys_ptr=f.sys_ptr)
+    plan = S.Plan(dP, dI, dV, f.n, f.nnz, substitution=int(bwd), flags=flags,
+                  sys_ptr=f.sys_ptr)
     db = S.DeviceArray.from_host(b)
     dx = S.DeviceArray(b.shape, np.float64)
     stream = S.Stream()
@@ -193,7 +254,8 @@ def main():
     kern_ms = float(np.mean(per))
     achieved = solve_bytes / (kern_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.form}_rhs{rhs}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.form}_rhs{rhs}"
+                         + ("_bwd" if bwd else "") + ".json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -212,7 +274,9 @@ def main():
         "dtype": "f64" if args.form == "plain" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOAD_DESC[args.workload] + f", nrhs={rhs}, "
+            "workload": WORKLOAD_DESC[args.workload]
+                        + (", upper triangle U = L^T, backward substitution" if bwd else "")
+                        + f", nrhs={rhs}, "
                         + ("FP64 sync-free" if args.form == "plain"
                            else f"FP32 solve + {args.ir_iters} FP64 refinement steps"),
             "n": f.n, "nnz": f.nnz, "nrhs": rhs, "dag_levels": levels,
@@ -232,8 +296,20 @@ def main():
         },
         "rel_err_vs_x_true": err_true,
     }
+    if args.workload == "batch" and args.form == "plain" and rhs == 1 and not bwd:
+        # the same 512 systems as 512 separate plain launches
+        ps_ms, dxs, spans = per_system_launches(f, b.reshape(-1))
+        worst = 0.0
+        for dx, (lo, hi) in zip(dxs, spans):
+            xs, xt = dx.to_host(), x_true.reshape(-1)[lo:hi]
+            worst = max(worst, float(np.abs(xs - xt).max() / np.abs(xt).max()))
+        out["per_system_launches"] = {
+            "ms": round(ps_ms, 4), "launches": len(dxs),
+            "batched_speedup": round(ps_ms / ms_step, 2),
+            "rel_err_vs_x_true_linf": worst,
+            "note": "same systems, one plan and one launch each, default executor"}
     if not args.no_cpu and rhs == 1:
-        cpu, x_cpu = cpu_baseline(f, b.reshape(-1), args.cpu_seconds)
+        cpu, x_cpu = cpu_baseline(f, b.reshape(-1), args.cpu_seconds, int(bwd))
         out["cpu_baseline"] = cpu
         # the CPU run is also the checker: same matrix, same b
         d = np.abs(x.reshape(-1) - x_cpu)

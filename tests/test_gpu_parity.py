@@ -395,3 +395,20 @@ def test_autotune_picks_a_working_config():
     B = f.spmv(mx.x_uniform(f.n, seed=3))
     x, _ = plan_solve(f, B, plan=plan)
     assert_close(x, oracle_x(f, B), 1e-12)
+
+
+# ------------------------------------------------------------ reference inputs
+@pytest.mark.parametrize("backward", [0, 1])
+def test_matrix_market_factor(tmp_path, backward):
+    import scipy.io
+    from sptrsv_amd import mtx
+    rng = np.random.default_rng(7)
+    A = sp.random(3000, 3000, density=0.002, random_state=rng, format="coo")
+    A = sp.tril(A + A.T + sp.eye(3000)).tocoo()
+    path = str(tmp_path / "a.mtx")
+    scipy.io.mmwrite(path, A, symmetry="symmetric")
+    f = mtx.triangle_from_mtx(path, substitution=backward, seed=1)
+    X = rng.integers(1, 4, size=f.n).astype(np.float64)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, subst=backward)
+    assert_close(x, oracle_x(f, B, subst=backward), 1e-12)
