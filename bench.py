@@ -140,6 +140,11 @@ def main():
                     help="backward solves U = L^T of the workload (diagonal last)")
     ap.add_argument("--mtx", default=None,
                     help="Matrix Market file: its unit-diagonal triangle (main.cu:152-187)")
+    ap.add_argument("--mtx-values", default="file", choices=["file", "random"],
+                    help="off-diagonal values of an --mtx factor: the file's, or the "
+                         "reference's random 1..10")
+    ap.add_argument("--mtx-diag", default="file", choices=["file", "unit"],
+                    help="diagonal of an --mtx factor: the file's, or the reference's 1")
     ap.add_argument("--csc", default=None, help="CSC text factor (main.cu:247-286)")
     ap.add_argument("--rhs", type=int, default=1)
     ap.add_argument("--form", default="plain", choices=["plain", "ir"])
@@ -179,7 +184,8 @@ def main():
     bwd = args.substitution == "backward"
     if args.mtx or args.csc:
         from sptrsv_amd import matrices as mx, mtx
-        f = (mtx.triangle_from_mtx(args.mtx, substitution=int(bwd)) if args.mtx
+        f = (mtx.triangle_from_mtx(args.mtx, substitution=int(bwd), values=args.mtx_values,
+                                   diag=args.mtx_diag) if args.mtx
              else mtx.read_csc_text(args.csc))
         x_true, levels = mx.x_uniform(f.n, seed=0), None
         args.workload = "file"

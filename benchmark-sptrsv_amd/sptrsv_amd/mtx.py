@@ -15,8 +15,11 @@
 
 The reference seeds ``rand()`` with the time of day (main.cu:140), so its
 values are not reproducible; here they come from ``numpy.random.default_rng``
-with an explicit seed.  ``values="file"`` keeps the file's values instead
-(off-diagonal entries only; the diagonal stays 1).
+with an explicit seed.  ``values="file"`` keeps the file's off-diagonal
+values instead, and ``diag="file"`` the file's diagonal instead of 1.  The
+reference's recipe (random values, unit diagonal) makes the solution grow
+like 10^(DAG depth) and overflow on large matrices; with the file's values a
+diagonally dominant matrix gives a well-conditioned factor.
 """
 import numpy as np
 
@@ -67,10 +70,12 @@ def read_mtx(path):
     return m, n, rows, cols, vals
 
 
-def triangle_from_mtx(path, substitution=FORWARD, values="random", seed=0):
-    """Unit-diagonal triangular factor of the matrix in a Matrix Market
-    file, as a CSC Factor (main.cu:152-187): strictly lower entries for
-    forward substitution, strictly upper for backward."""
+def triangle_from_mtx(path, substitution=FORWARD, values="random", seed=0,
+                      diag="unit"):
+    """Triangular factor of the matrix in a Matrix Market file, as a CSC
+    Factor (main.cu:152-187): strictly lower entries for forward
+    substitution, strictly upper for backward, plus the diagonal (1, or the
+    file's with diag="file")."""
     m, n, rows, cols, vals = read_mtx(path)
     if m != n:
         raise ValueError(f"{path}: matrix is {m} x {n}, not square")
@@ -81,8 +86,18 @@ def triangle_from_mtx(path, substitution=FORWARD, values="random", seed=0):
     elif values != "file":
         raise ValueError("values must be 'random' or 'file'")
     i = np.arange(n, dtype=np.int64)
+    if diag == "unit":
+        d = np.ones(n)
+    elif diag == "file":
+        on = rows == cols
+        d = np.zeros(n)
+        np.add.at(d, rows[on], vals[on])
+        if np.any(d == 0):
+            raise ValueError(f"{path}: {int(np.sum(d == 0))} rows have no (or a zero) diagonal")
+    else:
+        raise ValueError("diag must be 'unit' or 'file'")
     p, ri, vv = _from_columns(n, np.concatenate([c, i]), np.concatenate([r, i]),
-                              np.concatenate([v, np.ones(n)]))
+                              np.concatenate([v, d]))
     name = path.rsplit("/", 1)[-1] + ("_L" if substitution == FORWARD else "_U")
     return Factor(p, ri, vv, name=name)
 

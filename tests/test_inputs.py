@@ -111,3 +111,16 @@ def test_csc_text_roundtrip(tmp_path):
     (tmp_path / "short.csc").write_text("3 3 4\n1 2 3 5\n1 2\n")
     with pytest.raises(ValueError):
         mtx.read_csc_text(str(tmp_path / "short.csc"))
+
+
+def test_triangle_from_mtx_file_diagonal(tmp_path):
+    f0 = mx.laplacian2d_lower(12, 9)
+    A = sp.csc_matrix((f0.val, f0.rowIdx, f0.colPtr), shape=(f0.n, f0.n))
+    path = _write(tmp_path, "lap.mtx", sp.tril(A).tocoo(), symmetry="symmetric")
+    f = mtx.triangle_from_mtx(path, values="file", diag="file")
+    np.testing.assert_array_equal(f.colPtr, f0.colPtr)
+    np.testing.assert_array_equal(f.rowIdx, f0.rowIdx)
+    np.testing.assert_array_equal(f.val, f0.val)
+    nodiag = _write(tmp_path, "nd.mtx", sp.random(20, 20, density=0.2, random_state=1))
+    with pytest.raises(ValueError):
+        mtx.triangle_from_mtx(nodiag, diag="file")
