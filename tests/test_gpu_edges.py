@@ -1,0 +1,112 @@
+"""Edge cases of the executors on the GPU, against the oracle.
+
+Sizes at the lane engine's batch (64 columns) and window boundaries, a single
+column, a diagonal factor (no off-diagonal entries), an empty system, tiles
+wider than the right-hand-side count (rhs_per_lane > rhs), FP32 plans and
+batches of systems with several right-hand sides per lane, and hipGraph
+replays of such a solve.  Tolerances as in test_gpu_parity.py: FP64 1e-12,
+FP32 2e-5 relative to the serial oracle.
+"""
+import numpy as np
+import pytest
+
+import sptrsv_amd as S
+from sptrsv_amd import matrices as mx
+from gpu_util import assert_close, oracle_x, plan_solve, require_gpu
+
+pytestmark = pytest.mark.gpu
+
+LANE = S.ENGINE_LANE
+CFGS = [None, dict(lanes_per_column=4, block=256, window=64),
+        dict(engine=LANE, block=256, window=128), dict(engine=LANE, block=1024, window=8192)]
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+
+
+def _id(c):
+    if c is None:
+        return "default"
+    return ("lane" if c.get("engine") == LANE else "group") + "-B{block}-W{window}".format(**c)
+
+
+@pytest.mark.parametrize("cfg", CFGS, ids=_id)
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 8191, 8192, 8193, 16_385])
+def test_sizes_at_batch_and_window_boundaries(n, cfg):
+    f = mx.bidiagonal(n) if n < 100 else mx.power_law_lower(n, window=min(n, 500), seed=n)
+    b = f.spmv(mx.x_uniform(f.n, seed=n))
+    x, _ = plan_solve(f, b, cfg=cfg)
+    assert_close(x, oracle_x(f, b), 1e-12)
+
+
+@pytest.mark.parametrize("cfg", CFGS, ids=_id)
+@pytest.mark.parametrize("backward", [0, 1])
+def test_diagonal_factor(cfg, backward):
+    n = 3000
+    d = np.random.default_rng(3).uniform(1.0, 2.0, n)
+    f = mx.Factor(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    b = np.random.default_rng(4).uniform(-1, 1, n)
+    x, _ = plan_solve(f, b, subst=backward, cfg=cfg)
+    assert_close(x, b / d, 1e-15)
+
+
+def test_empty_system():
+    f = mx.Factor(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
+    db = S.DeviceArray((1,), np.float64)
+    dx = S.DeviceArray((1,), np.float64)
+    plan.solve(db, dx)
+    assert plan.check() == 0
+
+
+@pytest.mark.parametrize("R", [2, 4, 8])
+@pytest.mark.parametrize("rhs", [1, 3, 5])
+def test_lane_tile_wider_than_rhs(R, rhs):
+    f = mx.laplacian3d_lower(12, 10, 9)
+    X = mx.x_normal(f.n, rhs, seed=R) if rhs > 1 else mx.x_uniform(f.n, seed=R)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, rhs=rhs, cfg=dict(engine=LANE, rhs_per_lane=R, block=512, window=512))
+    assert_close(x, oracle_x(f, B, rhs=rhs), 1e-12)
+
+
+@pytest.mark.parametrize("R", [1, 4])
+def test_fp32_plan_multi_rhs_lane(R):
+    f = mx.laplacian2d_lower(70, 50)
+    X = mx.x_normal(f.n, 6, seed=9)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, rhs=6, dtype=np.float32,
+                      cfg=dict(engine=LANE, rhs_per_lane=R, block=256, window=1024))
+    assert_close(x, oracle_x(f, B, rhs=6, dtype=np.float32), 2e-5)
+
+
+@pytest.mark.parametrize("R", [2, 8])
+def test_batch_multi_rhs_lane(R):
+    f = mx.laplacian2d_batch(24, lo=8, hi=40, seed=5)
+    X = mx.x_normal(f.n, 5, seed=6)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, rhs=5, cfg=dict(engine=LANE, rhs_per_lane=R, block=256, window=900))
+    assert_close(x, oracle_x(f, B, rhs=5), 1e-12)
+
+
+def test_graph_replays_multi_rhs_lane():
+    f = mx.laplacian3d_lower(20, 16, 12)
+    X = mx.x_normal(f.n, 4, seed=11)
+    B = f.spmv(X)
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
+    plan.set_config(4, engine=LANE, rhs_per_lane=4, block=512, window=2048)
+    db = S.DeviceArray.from_host(B)
+    dx = S.DeviceArray(B.shape, np.float64)
+    plan.solve(db, dx, 4)
+    assert plan.check() == 0
+    ref = dx.to_host()
+    assert_close(ref, oracle_x(f, B, rhs=4), 1e-12)
+    stream = S.Stream()
+    g = plan.graph(db, dx, rhs=4, stream=stream)
+    for _ in range(3):
+        dx.zero()
+        g.launch(stream)
+        stream.synchronize()
+        assert plan.check(stream) == 0
+        assert_close(dx.to_host(), ref, 1e-12)
