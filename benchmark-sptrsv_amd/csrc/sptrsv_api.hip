@@ -378,6 +378,9 @@ struct WindowSet {
   DevBuf etgt;    // int[nnz]
   DevBuf rlist;   // int4[#columns with remote inputs]
   DevBuf chain;   // int2[n]: in-batch predecessor delivered by register
+  DevBuf bstart;  // int[#batches]: batch starts, window-local, by window
+  DevBuf boff;    // int[nwin+1]: each window's range in bstart
+  int maxnb = 1;  // most batches in one window
   DevBuf roff;    // int[nwin+1]
   int nslots = -1;
 };
@@ -418,8 +421,7 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   c.rhs_per_lane = rhs >= 4 ? 4 : rhs >= 2 ? 2 : 1;
   c.block = 1024;
   c.window = 8192;
-  const size_t per_col = (size_t)c.rhs_per_lane * 8 + sizeof(int);
-  while (c.window > 256 && ((size_t)c.window + 64) * per_col > p->lds_max)
+  while (c.window > 256 && lane_lds_bytes(c.window, c.rhs_per_lane, 8) > p->lds_max)
     c.window /= 2;
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
@@ -438,8 +440,7 @@ int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
     if (c.partition != SPTRSV_COL_PARTITION &&
         c.partition != SPTRSV_NNZ_PARTITION)
       return SPTRSV_ERR_INVALID_ARG;
-    if (((size_t)c.window + 64) * ((size_t)R * tsize + sizeof(int)) >
-        p->lds_max)
+    if (lane_lds_bytes(c.window, R, tsize) > p->lds_max)
       return SPTRSV_ERR_INVALID_ARG;
     return SPTRSV_OK;
   }
@@ -624,6 +625,79 @@ T slot_empty_host() {
   return v;
 }
 
+// Batches of a window partition: up to 64 consecutive columns.  In a window
+// where most columns continue a chain from the previous column (a stencil's
+// grid rows, chains), a batch also ends before a column that starts a new
+// chain once it holds kMinChainBatch columns, so grid rows whose length is
+// not a multiple of 64 still map to whole batches and a batch depends on the
+// batch above it, not on the one above and ahead.  Writes the kernel's
+// table (ws->bstart window-local, ws->boff) and, for the analysis, each
+// batch's first position (bfirst) and each position's batch (bid).
+int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
+                  hipStream_t s) {
+  const int n = p->n;
+  std::vector<int> wp(ws->nwin + 1, 0);
+  std::vector<unsigned char> link(std::max(n, 1), 0);
+  HIP_TRY(hipMemcpyAsync(wp.data(), ws->winPtr.p, sizeof(int) * wp.size(),
+                         hipMemcpyDeviceToHost, s));
+  if (n > 0) {
+    DevBuf dl;
+    ST_TRY(dl.ensure((size_t)n, false));
+    k_links<<<grid_for(n, 256), 256, 0, s>>>(
+        p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+        dl.as<unsigned char>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(link.data(), dl.p, (size_t)n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dl.release();
+    if (e != hipSuccess) return SPTRSV_ERR_HIP;
+  } else {
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  std::vector<int> local, first, owner(std::max(n, 1), 0), off(ws->nwin + 1, 0);
+  for (int w = 0; w < ws->nwin; w++) {
+    const int wlo = wp[w], whi = wp[w + 1];
+    off[w] = (int)local.size();
+    if (whi <= wlo) continue;
+    long links = 0;
+    for (int q = wlo + 1; q < whi; q++) links += link[q];
+    const bool chained = 2 * links >= (long)(whi - wlo - 1) && whi - wlo > 1;
+    int start = wlo;
+    local.push_back(0);
+    first.push_back(wlo);
+    for (int q = wlo; q < whi; q++) {
+      const int len = q - start;
+      if (q > wlo && (len >= kWave || (chained && !link[q] && len >= kMinChainBatch))) {
+        start = q;
+        local.push_back(q - wlo);
+        first.push_back(q);
+      }
+      owner[q] = (int)first.size() - 1;
+    }
+  }
+  off[ws->nwin] = (int)local.size();
+  ws->maxnb = 1;
+  for (int w = 0; w < ws->nwin; w++) ws->maxnb = std::max(ws->maxnb, off[w + 1] - off[w]);
+  const size_t nb = std::max<size_t>(local.size(), 1);
+  ST_TRY(ws->bstart.ensure(sizeof(int) * nb, false));
+  ST_TRY(ws->boff.ensure(sizeof(int) * off.size(), false));
+  ST_TRY(bfirst->ensure(sizeof(int) * nb, false));
+  ST_TRY(bid->ensure(sizeof(int) * owner.size(), false));
+  if (!local.empty()) {
+    HIP_TRY(hipMemcpyAsync(ws->bstart.p, local.data(), sizeof(int) * local.size(),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(bfirst->p, first.data(), sizeof(int) * first.size(),
+                           hipMemcpyHostToDevice, s));
+  }
+  HIP_TRY(hipMemcpyAsync(ws->boff.p, off.data(), sizeof(int) * off.size(),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(bid->p, owner.data(), sizeof(int) * owner.size(),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return SPTRSV_OK;
+}
+
 // Lane-engine analysis of a window partition: slot ranges per row (exclusive
 // scan of the remote arrival counts) and the slot or LDS target of every
 // off-diagonal entry.
@@ -657,32 +731,34 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     cursor.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
   }
-  // in-batch predecessors delivered by register (must follow k_lane_targets,
-  // whose etgt values it overwrites for the selected entries)
+  // batches and in-batch predecessors delivered by register (must follow
+  // k_lane_targets, whose etgt values it overwrites for the selected entries)
   ST_TRY(ws->chain.ensure(sizeof(int2) * (size_t)std::max(n, 1), false));
+  DevBuf bfirst, bid;
+  ST_TRY(build_batches(p, ws, &bfirst, &bid, s));
   if (n > 0) {
     DevBuf key;
     ST_TRY(key.ensure(sizeof(unsigned long long) * (size_t)n, true));
     const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
     if (p->vt == SPTRSV_F64)
       k_chain_key<double><<<grid_for(n, 256), 256, 0, s>>>(
-          p->colPtr, p->rowIdx, (const double *)p->val, n, bwd,
-          ws->winPtr.as<int>(), ws->nwin, ws->uniform_w,
+          p->colPtr, p->rowIdx, (const double *)p->val, n, bwd, bid.as<int>(),
           key.as<unsigned long long>());
     else
       k_chain_key<float><<<grid_for(n, 256), 256, 0, s>>>(
-          p->colPtr, p->rowIdx, (const float *)p->val, n, bwd,
-          ws->winPtr.as<int>(), ws->nwin, ws->uniform_w,
+          p->colPtr, p->rowIdx, (const float *)p->val, n, bwd, bid.as<int>(),
           key.as<unsigned long long>());
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
       k_chain_mark<<<grid_for(n, 256), 256, 0, s>>>(
-          key.as<unsigned long long>(), n, bwd, ws->winPtr.as<int>(), ws->nwin,
-          ws->uniform_w, ws->etgt.as<int>(), ws->chain.as<int2>());
+          key.as<unsigned long long>(), n, bfirst.as<int>(), bid.as<int>(),
+          ws->etgt.as<int>(), ws->chain.as<int2>());
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     key.release();
+    bfirst.release();
+    bid.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
   }
   // receive lists: the columns with remote inputs, grouped by window
@@ -774,8 +850,11 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     ST_TRY(get_lane_targets(p, c, L->ws, s));
     L->R = c.rhs_per_lane;
     L->ntiles = (rhs + L->R - 1) / L->R;
-    // window rows plus one dummy row per lane
-    L->lds = ((size_t)L->ws->maxw + 64) * ((size_t)L->R * sizeof(T) + sizeof(int));
+    // window rows plus one dummy row per lane, then the batch starts
+    L->lds = ((size_t)L->ws->maxw + 64) * ((size_t)L->R * sizeof(T) + sizeof(int)) +
+             sizeof(int) * (size_t)L->ws->maxnb;
+    if (L->lds > lane_lds_bytes(L->ws->maxw, L->R, sizeof(T)))
+      return SPTRSV_ERR_INVALID_ARG;  // cannot happen: maxnb <= maxw/16 + 1
     DevBuf &sb = slot_buf<T>(p);
     const size_t want =
         std::max<size_t>(1, (size_t)L->ntiles * L->ws->nslots * L->R) *
@@ -846,6 +925,9 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.chain = L.ws->chain.as<int2>();
     la.rptr = L.ws->rptr.as<int>();
     la.winPtr = L.ws->winPtr.as<int>();
+    la.bstart = L.ws->bstart.as<int>();
+    la.boff = L.ws->boff.as<int>();
+    la.maxnb = L.ws->maxnb;
     la.rlist = L.ws->rlist.as<int4>();
     la.roff = L.ws->roff.as<int>();
     la.nrecv = lane_receivers(c.block);
@@ -1095,9 +1177,7 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     if (R > 1 && R >= 2 * rhs) break;  // no tile narrower than half of R
     int added = 0;  // the two widest windows that fit
     for (int window : {8192, 4096, 2048, 1024}) {
-      if (((size_t)window + 64) * ((size_t)R * tsize + sizeof(int)) >
-          p->lds_max)
-        continue;
+      if (lane_lds_bytes(window, R, tsize) > p->lds_max) continue;
       for (int block : {1024, 512, 256})
         add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
       if (++added == 2) break;
@@ -1438,6 +1518,8 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     kv.second->rlist.release();
     kv.second->roff.release();
     kv.second->chain.release();
+    kv.second->bstart.release();
+    kv.second->boff.release();
     delete kv.second;
   }
   p->windows.clear();
@@ -1556,7 +1638,8 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
     bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes +
                          kv.second->rptr.bytes + kv.second->etgt.bytes +
                          kv.second->rlist.bytes + kv.second->roff.bytes +
-                         kv.second->chain.bytes);
+                         kv.second->chain.bytes + kv.second->bstart.bytes +
+                         kv.second->boff.bytes);
   info->scratch_bytes = bytes;
   info->analysis_ms = p->analysis_ms;
   return SPTRSV_OK;

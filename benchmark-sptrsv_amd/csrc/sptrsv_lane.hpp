@@ -64,6 +64,8 @@ struct LaneArgs {
                                   // or -1, its entry index}
   const int *__restrict__ rptr;   // [n+1] slot range of each row
   const int *__restrict__ winPtr; // [nwin + 1] window bounds, solve order
+  const int *__restrict__ bstart; // batch starts, window-local, by window
+  const int *__restrict__ boff;   // [nwin + 1] each window's bstart range
   const int4 *__restrict__ rlist; // columns with remote inputs, window order:
                                   // {local index, first slot, slot count, -}
   const int *__restrict__ roff;   // [nwin + 1] each window's rlist range
@@ -71,6 +73,7 @@ struct LaneArgs {
   const T *__restrict__ b;        // [n][rhs]
   T *__restrict__ x;              // [n][rhs]
   int n, rhs, nwin, ntiles, maxw, nslots;
+  int maxnb;                      // most batches in one window
   int nrecv;                      // receiver wavefronts per workgroup
   int *ticket;
   int *err;
@@ -148,8 +151,20 @@ __device__ __forceinline__ bool chain_ready(unsigned long long linked,
 // row r depends on, when |L(r, c)| <= |L(r, r)|.
 constexpr int kRegTarget = -2147483647 - 1;
 
-// Compute wavefront cw handles batches of 64 consecutive columns (lane =
-// column) cw, cw + ncw, ..., each column for the R right-hand sides of its
+// A batch that ends before a column starting a new chain holds at least this
+// many columns (so a window has at most window / kMinChainBatch + 1 batches).
+constexpr int kMinChainBatch = 16;
+
+// LDS of a lane-engine workgroup: window rows plus one dummy row per lane
+// (R values and a counter each), then the window's batch starts.
+__host__ __device__ inline size_t lane_lds_bytes(size_t window, int R,
+                                                 size_t tsize) {
+  return (window + kWave) * ((size_t)R * tsize + sizeof(int)) +
+         sizeof(int) * (window / kMinChainBatch + 2);
+}
+
+// Compute wavefront cw handles batches of up to 64 consecutive columns (lane
+// = column) cw, cw + ncw, ..., each column for the R right-hand sides of its
 // tile (rv of them valid).  Each row r may have one in-batch predecessor p(r)
 // whose entry is delivered by register (analysis: `chain`); every other input
 // arrives through LDS (same window) or a slot (earlier window, folded into
@@ -167,16 +182,21 @@ constexpr int kRegTarget = -2147483647 - 1;
 // batch always drains.
 template <typename T, bool BWD, int R>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
-                                            int *s_cnt, int wlo, int wlen,
+                                            int *s_cnt, const int *s_bst,
+                                            int nb, int wlo, int wlen,
                                             int tile, int rv, int cw,
                                             int ncw) {
   const int lane = __lane_id();
   const int n = a.n;
   T *slots = a.slots + (size_t)tile * a.nslots * R;
   bool stopped = false;  // the stall watch fired: leave the window
-  for (int base = cw * kWave; base < wlen && !stopped; base += ncw * kWave) {
+  for (int bi = cw; bi < nb && !stopped; bi += ncw) {
+    // a batch: up to 64 consecutive columns (window-local [base, bend)),
+    // bounds staged in LDS
+    const int base = s_bst[bi];
+    const int bend = bi + 1 < nb ? s_bst[bi + 1] : wlen;
     const int k = base + lane;
-    bool active = k < wlen;
+    bool active = k < bend;
     int lo = 0, hi = 0, need_lds = 0, ne = 0, pl = lane;
     T diag = T(1), inv = T(1), coef = T(0);
     int tg[kPF];
@@ -438,8 +458,9 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
   // maxw window rows plus 64 dummy rows (one per lane) for branch-free
-  // publishing
+  // publishing, then the window's batch starts
   int *s_cnt = reinterpret_cast<int *>(s_val + (size_t)(a.maxw + kWave) * R);
+  int *s_bst = s_cnt + a.maxw + kWave;
   __shared__ int s_ticket;
   const int total = a.nwin * a.ntiles;
   const int wave = threadIdx.x / kWave;
@@ -465,12 +486,14 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
       s_val[i] = q < rv ? a.b[(size_t)c * a.rhs + tile * R + q] : T(0);
     }
     for (int k = threadIdx.x; k < wlen; k += blockDim.x) s_cnt[k] = 0;
+    const int b0 = a.boff[win], nb = a.boff[win + 1] - b0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s_bst[i] = a.bstart[b0 + i];
     __syncthreads();
     if (wave < a.nrecv)
       lane_receive<T, R>(a, s_val, s_cnt, win, tile, rv, wave);
     else
-      lane_compute<T, BWD, R>(a, s_val, s_cnt, wlo, wlen, tile, rv,
-                              wave - a.nrecv, ncw);
+      lane_compute<T, BWD, R>(a, s_val, s_cnt, s_bst, nb, wlo, wlen, tile,
+                              rv, wave - a.nrecv, ncw);
     __syncthreads();
     for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
       const int k = R == 1 ? i : i / R;
@@ -515,30 +538,49 @@ __global__ void k_lane_targets(const int *__restrict__ colPtr,
   }
 }
 
+// Chain links by solve position: link[p] = 1 when the column at position
+// p - 1 has an entry in the row at position p (the nearest off-diagonal
+// entry: first after the diagonal forward, last before it backward).
+__global__ void k_links(const int *__restrict__ colPtr,
+                        const int *__restrict__ rowIdx, int n, int backward,
+                        unsigned char *link) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += gridDim.x * blockDim.x) {
+    unsigned char v = 0;
+    if (p > 0) {
+      const int c = backward ? n - p : p - 1;       // column at position p - 1
+      const int want = backward ? n - 1 - p : p;    // row at position p
+      const int f = colPtr[c], l = colPtr[c + 1];
+      if (backward)
+        v = l - 2 >= f && rowIdx[l - 2] == want;
+      else
+        v = f + 1 < l && rowIdx[f + 1] == want;
+    }
+    link[p] = v;
+  }
+}
+
 // In-batch predecessor selection: for every row, the nearest earlier column
-// of the same 64-column batch (solve order) among those with |L(r, c)| <=
-// |L(r, r)|, as key = (position + 1) << 32 | entry index, kept by atomicMax
-// (key zeroed).  The bound keeps the products of the affine coefficients of
-// a jumped chain at most 1 in magnitude.
+// of the same batch (solve order) among those with |L(r, c)| <= |L(r, r)|,
+// as key = (position + 1) << 32 | entry index, kept by atomicMax (key
+// zeroed).  The bound keeps the products of the affine coefficients of a
+// jumped chain at most 1 in magnitude.
 template <typename T>
 __global__ void k_chain_key(const int *__restrict__ colPtr,
                             const int *__restrict__ rowIdx,
                             const T *__restrict__ val, int n, int backward,
-                            const int *__restrict__ winPtr, int nwin,
-                            int uniform_w, unsigned long long *key) {
+                            const int *__restrict__ bid,
+                            unsigned long long *key) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += gridDim.x * blockDim.x) {
     const int p = backward ? n - 1 - c : c;
-    const int w = uniform_w > 0 ? p / uniform_w
-                                : upper_bound_minus1(winPtr, nwin + 1, p);
-    const int wlo = winPtr[w], whi = winPtr[w + 1];
-    const int lb = (p - wlo) >> 6;
+    const int lb = bid[p];
     const int f = colPtr[c], l = colPtr[c + 1];
     const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
     for (int j = lo; j < hi; j++) {
       const int r = rowIdx[j];
       const int pr = backward ? n - 1 - r : r;
-      if (pr >= whi || ((pr - wlo) >> 6) != lb) continue;
+      if (bid[pr] != lb) continue;
       const T d = val[backward ? colPtr[r + 1] - 1 : colPtr[r]];
       const T v = val[j];
       if (!(fabs(v) <= fabs(d))) continue;
@@ -548,10 +590,11 @@ __global__ void k_chain_key(const int *__restrict__ colPtr,
 }
 
 // chain[r] = {lane of the selected predecessor, its entry} or {-1, 0}; the
-// selected entry's target becomes kRegTarget.
+// selected entry's target becomes kRegTarget.  bfirst: first solve position
+// of each batch; bid: batch of each solve position.
 __global__ void k_chain_mark(const unsigned long long *__restrict__ key, int n,
-                             int backward, const int *__restrict__ winPtr,
-                             int nwin, int uniform_w, int *etgt, int2 *chain) {
+                             const int *__restrict__ bfirst, const int *__restrict__ bid,
+                             int *etgt, int2 *chain) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += gridDim.x * blockDim.x) {
     const unsigned long long kk = key[r];
@@ -561,9 +604,7 @@ __global__ void k_chain_mark(const unsigned long long *__restrict__ key, int n,
     }
     const int pc = (int)(kk >> 32) - 1;
     const int j = (int)(unsigned)(kk & 0xffffffffull);
-    const int w = uniform_w > 0 ? pc / uniform_w
-                                : upper_bound_minus1(winPtr, nwin + 1, pc);
-    chain[r] = make_int2((pc - winPtr[w]) & (kWave - 1), j);
+    chain[r] = make_int2(pc - bfirst[bid[pc]], j);  // lane in its batch
     etgt[j] = kRegTarget;
   }
 }
