@@ -328,17 +328,30 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 #pragma unroll
               for (int q = 0; q < R; q++)
                 add_wg(&s_val[ladr[e] * R + q], -(av[q] * lval[e]));
-            for (int j = lo + kPF; j < hi; j++) {
-              const int t = a.etgt[j];
-              const T v = a.val[j];
-              if (t >= 0) {
+            // entries past the registers, kPF at a time (loads issued
+            // together, one wait per group)
+            for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
+              int tt[kPF];
+              T tv[kPF];
 #pragma unroll
-                for (int q = 0; q < R; q++)
-                  if (q < rv) xchg_agent(&slots[(size_t)t * R + q], av[q] * v);
-              } else if (t != kRegTarget) {
+              for (int i = 0; i < kPF; i++) {
+                tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
+                tv[i] = j0 + i < hi ? a.val[j0 + i] : T(0);
+              }
 #pragma unroll
-                for (int q = 0; q < R; q++)
-                  if (q < rv) add_wg(&s_val[(-t - 1) * R + q], -(av[q] * v));
+              for (int i = 0; i < kPF; i++) {
+                const int t = tt[i];
+                if (t >= 0) {
+#pragma unroll
+                  for (int q = 0; q < R; q++)
+                    if (q < rv)
+                      xchg_agent(&slots[(size_t)t * R + q], av[q] * tv[i]);
+                } else if (t != kRegTarget) {
+#pragma unroll
+                  for (int q = 0; q < R; q++)
+                    if (q < rv)
+                      add_wg(&s_val[(-t - 1) * R + q], -(av[q] * tv[i]));
+                }
               }
             }
           }
@@ -346,9 +359,14 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           if (solved) {
 #pragma unroll
             for (int e = 0; e < kPF; e++) add_wg(&s_cnt[ladr[e]], linc[e]);
-            for (int j = lo + kPF; j < hi; j++) {
-              const int t = a.etgt[j];
-              if (t < 0 && t != kRegTarget) add_wg(&s_cnt[-t - 1], 1);
+            for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
+              int tt[kPF];
+#pragma unroll
+              for (int i = 0; i < kPF; i++)
+                tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
+#pragma unroll
+              for (int i = 0; i < kPF; i++)
+                if (tt[i] < 0 && tt[i] != kRegTarget) add_wg(&s_cnt[-tt[i] - 1], 1);
             }
           }
         }
@@ -375,7 +393,10 @@ constexpr int kTake = 8;  // slots of one row taken per group
 // in groups of up to kTake with exchange(slot, EMPTY), which both polls and
 // resets a slot.  Only the first slot of a group is polled until it arrives;
 // the rest of the group is taken after it, and the group is summed in slot
-// order once complete, so the remote sum has a fixed order.  A receiver
+// order once complete.  A row with more than kRecvChunk remote slots has one
+// receive-list entry per kRecvChunk of them, so its chunks are taken by
+// several lanes at once and added to the row's LDS sum (and count) one by
+// one.  A receiver
 // wavefront that took nothing in an iteration backs off.
 template <typename T, int R>
 __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
@@ -615,16 +636,23 @@ __global__ void k_need_remote(const int2 *__restrict__ need, int n, int *out) {
     out[r] = need[r].y;
 }
 
-// flag[p] = 1 when the column at solve position p has remote inputs.
+// A receive-list entry covers at most kRecvChunk slots of a row, so the
+// slots of a dense row are taken by several receiver lanes in parallel
+// instead of one lane group by group.
+constexpr int kRecvChunk = 2 * kTake;
+
+// count[p] = receive-list entries of the column at solve position p.
 __global__ void k_remote_flags(const int2 *__restrict__ need, int n,
-                               int backward, int *flag) {
+                               int backward, int *count) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += gridDim.x * blockDim.x)
-    flag[p] = need[backward ? n - 1 - p : p].y > 0 ? 1 : 0;
+    count[p] = (need[backward ? n - 1 - p : p].y + kRecvChunk - 1) / kRecvChunk;
 }
 
-// Receive lists: pos = exclusive scan of the flags (solve order), so each
-// window's entries are contiguous and ascending; roff[w] = pos[winPtr[w]].
+// Receive lists: pos = exclusive scan of the entry counts (solve order), so
+// each window's entries are contiguous and ascending; roff[w] =
+// pos[winPtr[w]].  Entry i of a column covers its slots
+// [rptr + i * kRecvChunk, ...), at most kRecvChunk of them.
 __global__ void k_receive_list(const int2 *__restrict__ need,
                                const int *__restrict__ rptr,
                                const int *__restrict__ pos,
@@ -638,7 +666,9 @@ __global__ void k_receive_list(const int2 *__restrict__ need,
     if (m > 0) {
       const int w = uniform_w > 0 ? p / uniform_w
                                   : upper_bound_minus1(winPtr, nwin + 1, p);
-      rlist[pos[p]] = make_int4(p - winPtr[w], rptr[c], m, 0);
+      for (int i = 0, s = 0; s < m; i++, s += kRecvChunk)
+        rlist[pos[p] + i] =
+            make_int4(p - winPtr[w], rptr[c] + s, min(kRecvChunk, m - s), 0);
     }
   }
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= nwin; w += stride)
