@@ -16,6 +16,7 @@
 #include "sptrsv_exec.hpp"
 #include "sptrsv_ir.hpp"
 #include "sptrsv_lane.hpp"
+#include "sptrsv_row.hpp"
 #include "sptrsv_transpose.hpp"
 
 using namespace sptrsv;
@@ -27,6 +28,9 @@ int g_bench_repeat = 100;  // BENCH_REPEAT, SpTRSV_cuda/common.h:13-14
 // longest single spin-wait before a solve gives up, in microseconds: far
 // beyond any legitimate wait of a valid factor
 constexpr long long kDefaultWaitCapUs = 30ll * 1000 * 1000;
+// an off-diagonal entry this many columns from the diagonal or more lies
+// outside the default lane window of any column
+constexpr int kFarDistance = 8192;
 
 #define HIP_TRY(expr)                                                     \
   do {                                                                    \
@@ -396,8 +400,11 @@ struct sptrsv_plan_s {
   DevBuf slots64, slots32;  // lane-engine slots, kept at the empty sentinel
   std::map<long long, WindowSet *> windows;     // key: window*2 + partition
   std::map<long long, sptrsv_config_t> configs;  // key: rhs*2 + is_f32_solve
-  // mixed precision
-  DevBuf val32, csrPtr, csrIdx, csrVal, xtmp, r32, d32;
+  // mixed precision; the CSR (values in the plan's type) also serves the row
+  // engine, csrVal32 its FP32 solves of an FP64 plan
+  DevBuf val32, csrPtr, csrIdx, csrVal, xtmp, r32, d32, csrVal32;
+  DevBuf farcnt;  // unsigned long long: entries >= kFarDistance off the diagonal
+  bool csr_ready = false;
   long long wait_cap_us = kDefaultWaitCapUs;
   long long clock_khz = 100000;  // wall_clock64 rate (queried at creation)
   int cus = 0;
@@ -405,6 +412,7 @@ struct sptrsv_plan_s {
   double analysis_ms = 0.0;
   std::vector<int> sys_ptr;
   double avg_offdiag = 0.0;
+  double far_frac = 0.0;  // off-diagonal entries >= kFarDistance from the diagonal
 };
 
 namespace {
@@ -412,6 +420,17 @@ namespace {
 sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   sptrsv_config_t c;
   memset(&c, 0, sizeof(c));
+  if (p->far_frac >= 0.5) {
+    // most entries cross any LDS window: pull over the CSR, a wavefront per
+    // row (power-law hubs have hundreds of entries)
+    c.engine = SPTRSV_ENGINE_ROW;
+    c.lanes_per_column = 64;
+    c.rhs_lanes = 1;
+    c.rhs_per_lane = 1;
+    c.block = 256;
+    c.window = 1;
+    return c;
+  }
   // lane engine: one lane per column carrying up to 4 right-hand sides,
   // the widest LDS window that fits (8192 columns at one right-hand side),
   // 16 wavefronts
@@ -429,6 +448,19 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
 }
 
 int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
+  if (c.engine == SPTRSV_ENGINE_ROW) {
+    // lanes_per_column = lanes per row: 4, 16 or 64; window and partition
+    // are unused
+    (void)p;
+    (void)tsize;
+    const int G = c.lanes_per_column;
+    if (!(G == 4 || G == 16 || G == 64) || c.rhs_lanes != 1 ||
+        c.rhs_per_lane != 1)
+      return SPTRSV_ERR_INVALID_ARG;
+    if (c.block < 64 || c.block > 1024 || c.block % 64 != 0)
+      return SPTRSV_ERR_INVALID_ARG;
+    return SPTRSV_OK;
+  }
   if (c.engine == SPTRSV_ENGINE_LANE) {
     // rhs_per_lane = right-hand sides each lane carries (1, 2, 4 or 8)
     const int R = c.rhs_per_lane;
@@ -835,7 +867,61 @@ struct Launch {
   WindowSet *ws = nullptr;
   int ntiles = 0, R = 0, grid = 0;
   size_t lds = 0;
+  const void *rowval = nullptr;  // row engine: CSR values of the solve type
 };
+
+// CSR of the factor (values in the plan's type), built once by the device
+// transpose: the CSC of L read as the CSR of L^T, transposed.
+int ensure_csr(sptrsv_plan_s *p, hipStream_t s) {
+  if (p->csr_ready) return SPTRSV_OK;
+  const size_t tsize = p->vt == SPTRSV_F64 ? 8 : 4;
+  ST_TRY(p->csrPtr.ensure(sizeof(int) * ((size_t)p->n + 1), false));
+  ST_TRY(p->csrIdx.ensure(sizeof(int) * std::max(p->nnz, 1), false));
+  ST_TRY(p->csrVal.ensure(tsize * std::max(p->nnz, 1), false));
+  ST_TRY(csr2csc_impl(p->n, p->n, p->nnz, p->colPtr, p->rowIdx, p->val, p->vt,
+                      p->csrPtr.as<int>(), p->csrIdx.as<int>(), p->csrVal.p,
+                      s));
+  p->csr_ready = true;
+  return SPTRSV_OK;
+}
+
+// CSR values in the solve type T (an FP32 solve of an FP64 plan reads a
+// rounded copy).
+template <typename T>
+int row_values(sptrsv_plan_s *p, hipStream_t s, const T **out) {
+  ST_TRY(ensure_csr(p, s));
+  if ((sizeof(T) == 8) == (p->vt == SPTRSV_F64)) {
+    *out = p->csrVal.as<T>();
+    return SPTRSV_OK;
+  }
+  if (sizeof(T) != 4) return SPTRSV_ERR_UNSUPPORTED;
+  if (!p->csrVal32.p) {
+    ST_TRY(p->csrVal32.ensure(sizeof(float) * std::max(p->nnz, 1), false));
+    if (p->nnz > 0) {
+      k_to_f32<<<grid_for(p->nnz, 256), 256, 0, s>>>(
+          p->csrVal.as<double>(), p->csrVal32.as<float>(), p->nnz);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  *out = reinterpret_cast<const T *>(p->csrVal32.p);
+  return SPTRSV_OK;
+}
+
+template <typename T>
+using RowKernel = void (*)(RowArgs<T>);
+
+template <typename T>
+RowKernel<T> pick_row(bool bwd, int G) {
+  switch (G * 2 + (bwd ? 1 : 0)) {
+    case 8: return k_exec_row<T, false, 4>;
+    case 9: return k_exec_row<T, true, 4>;
+    case 32: return k_exec_row<T, false, 16>;
+    case 33: return k_exec_row<T, true, 16>;
+    case 128: return k_exec_row<T, false, 64>;
+    case 129: return k_exec_row<T, true, 64>;
+    default: return nullptr;
+  }
+}
 
 template <typename T>
 int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
@@ -845,6 +931,28 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
   L->cfg = it != p->configs.end() ? it->second : default_config(p, rhs);
   const sptrsv_config_t &c = L->cfg;
   ST_TRY(check_config(p, c, sizeof(T)));
+  if (c.engine == SPTRSV_ENGINE_ROW) {
+    const T *rv = nullptr;
+    ST_TRY(row_values<T>(p, s, &rv));
+    L->rowval = rv;
+    L->R = 1;
+    L->ntiles = 1;
+    RowKernel<T> kern =
+        pick_row<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, c.lanes_per_column);
+    if (!kern) return SPTRSV_ERR_INVALID_ARG;
+    // the static item assignment needs every workgroup resident: the grid
+    // never exceeds the resident capacity
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern,
+                                                         c.block, 0));
+    const int cap = std::max(1, per_cu) * p->cus;
+    const long long groups_needed =
+        ((long long)p->n * rhs + (c.block / c.lanes_per_column) - 1) /
+        (c.block / c.lanes_per_column);
+    int grid = c.grid > 0 ? std::min(c.grid, cap) : cap;
+    L->grid = (int)std::max(1LL, std::min<long long>(grid, groups_needed));
+    return SPTRSV_OK;
+  }
   ST_TRY(get_windows(p, c, &L->ws, s));
   if (c.engine == SPTRSV_ENGINE_LANE) {
     ST_TRY(get_lane_targets(p, c, L->ws, s));
@@ -916,6 +1024,26 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
                 T *x, int rhs, hipStream_t s) {
   if (p->n == 0) return SPTRSV_OK;
   const sptrsv_config_t &c = L.cfg;
+  if (c.engine == SPTRSV_ENGINE_ROW) {
+    // x starts at the sentinel: each solution is its own arrival flag
+    ST_TRY(fill_async<T>(x, (size_t)p->n * rhs, slot_empty_host<T>(), s));
+    RowArgs<T> ra;
+    ra.rowPtr = p->csrPtr.as<int>();
+    ra.colIdx = p->csrIdx.as<int>();
+    ra.val = reinterpret_cast<const T *>(L.rowval);
+    ra.b = b;
+    ra.x = x;
+    ra.n = p->n;
+    ra.rhs = rhs;
+    ra.err = p->sync.as<int>() + 2;
+    ra.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
+    RowKernel<T> kern =
+        pick_row<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, c.lanes_per_column);
+    if (!kern) return SPTRSV_ERR_INVALID_ARG;
+    kern<<<L.grid, c.block, 0, s>>>(ra);
+    HIP_TRY(hipGetLastError());
+    return SPTRSV_OK;
+  }
   if (c.engine == SPTRSV_ENGINE_LANE) {
     LaneArgs<T> la;
     la.colPtr = p->colPtr;
@@ -1068,6 +1196,7 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
   if (st == SPTRSV_OK && (hipEventCreate(&e0) != hipSuccess ||
                           hipEventCreate(&e1) != hipSuccess))
     st = SPTRSV_ERR_HIP;
+  unsigned long long far_entries = 0;
   if (st == SPTRSV_OK) st = p->sync.ensure(4 * sizeof(int), true);
   if (st == SPTRSV_OK) st = p->inDegree.ensure(sizeof(int) * std::max(n, 1), false);
   if (st == SPTRSV_OK && hipEventRecord(e0, s) != hipSuccess) st = SPTRSV_ERR_HIP;
@@ -1079,6 +1208,16 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
       k_histogram<<<grid_for(nnz, 256), 256, 0, s>>>(rowIdx, nnz,
                                                      p->inDegree.as<int>());
       if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
+    }
+    // entries farther from the diagonal than the default lane window
+    if (st == SPTRSV_OK && nnz > n) st = p->farcnt.ensure(sizeof(unsigned long long), true);
+    if (st == SPTRSV_OK && nnz > n) {
+      k_count_far<<<std::min(grid_for(n, 256), 2048), 256, 0, s>>>(
+          colPtr, rowIdx, n, kFarDistance, p->farcnt.as<unsigned long long>());
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(&far_entries, p->farcnt.p, sizeof(far_entries),
+                         hipMemcpyDeviceToHost, s) != hipSuccess)
+        st = SPTRSV_ERR_HIP;
     }
   }
   if (st == SPTRSV_OK && (flags & SPTRSV_PLAN_MIXED) && n > 0) {
@@ -1096,10 +1235,12 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
       st = csr2csc_impl(n, n, nnz, colPtr, rowIdx, val, SPTRSV_F64,
                         p->csrPtr.as<int>(), p->csrIdx.as<int>(),
                         p->csrVal.p, s);
+    if (st == SPTRSV_OK) p->csr_ready = true;
   }
   if (st == SPTRSV_OK && hipEventRecord(e1, s) != hipSuccess) st = SPTRSV_ERR_HIP;
   if (st == SPTRSV_OK && hipEventSynchronize(e1) != hipSuccess) st = SPTRSV_ERR_HIP;
   if (st == SPTRSV_OK) {
+    p->far_frac = nnz > n ? (double)far_entries / (double)(nnz - n) : 0.0;
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     p->analysis_ms = ms;
@@ -1183,6 +1324,8 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
       if (++added == 2) break;
     }
   }
+  for (int G : {4, 16, 64})
+    for (int block : {256, 1024}) add(G, 1, 1, block, 1, 0, SPTRSV_ENGINE_ROW);
   if (rhs <= 1) {
     for (int G : {1, 2, 4, 8, 16, 64})
       for (int block : {256, 1024}) {
@@ -1526,7 +1669,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
   DevBuf *bufs[] = {&p->inDegree, &p->sync,    &p->gsum,    &p->gcnt,
                     &p->slots64,  &p->slots32, &p->val32,   &p->csrPtr,
                     &p->csrIdx,   &p->csrVal,  &p->xtmp,    &p->r32,
-                    &p->d32};
+                    &p->d32,      &p->csrVal32, &p->farcnt};
   for (DevBuf *b : bufs) b->release();
   delete p;
   return SPTRSV_OK;
@@ -1536,6 +1679,13 @@ int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg)
   if (!p || !cfg || rhs < 1) return SPTRSV_ERR_INVALID_ARG;
   sptrsv_config_t c = *cfg;
   const sptrsv_config_t d = default_config(p, rhs);
+  if (c.engine == SPTRSV_ENGINE_ROW) {
+    if (c.lanes_per_column == 0) c.lanes_per_column = 64;
+    if (c.rhs_lanes == 0) c.rhs_lanes = 1;
+    if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
+    if (c.block == 0) c.block = 256;
+    if (c.window == 0) c.window = 1;
+  }
   if (c.engine == SPTRSV_ENGINE_LANE) {
     if (c.lanes_per_column == 0) c.lanes_per_column = 1;
     if (c.rhs_lanes == 0) c.rhs_lanes = 1;
@@ -1632,7 +1782,7 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
   const DevBuf *bufs[] = {&p->inDegree, &p->sync,    &p->gsum,  &p->gcnt,
                           &p->slots64,  &p->slots32, &p->val32, &p->csrPtr,
                           &p->csrIdx,   &p->csrVal,  &p->xtmp,  &p->r32,
-                          &p->d32};
+                          &p->d32,      &p->csrVal32, &p->farcnt};
   for (const DevBuf *b : bufs) bytes += (long long)b->bytes;
   for (auto &kv : p->windows)
     bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes +

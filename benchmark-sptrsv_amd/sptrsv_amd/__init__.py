@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (COL_PARTITION, ENGINE_GROUP, ENGINE_LANE, EXPORTS, F32, F64,
+from ._lib import (COL_PARTITION, ENGINE_GROUP, ENGINE_LANE, ENGINE_ROW, EXPORTS, F32, F64,
                    LIB_PATH, NNZ_PARTITION,
                    OPT_WARP_AUTO, OPT_WARP_NNZ, OPT_WARP_RHS, PLAN_MIXED,
                    PLAN_VALIDATE, STATUS, SUBSTITUTION_BACKWARD,
@@ -19,7 +19,7 @@ from ._lib import (COL_PARTITION, ENGINE_GROUP, ENGINE_LANE, EXPORTS, F32, F64,
                    lib)
 
 __all__ = [
-    "COL_PARTITION", "NNZ_PARTITION", "ENGINE_GROUP", "ENGINE_LANE", "OPT_WARP_AUTO", "OPT_WARP_NNZ",
+    "COL_PARTITION", "NNZ_PARTITION", "ENGINE_GROUP", "ENGINE_LANE", "ENGINE_ROW", "OPT_WARP_AUTO", "OPT_WARP_NNZ",
     "OPT_WARP_RHS", "SUBSTITUTION_BACKWARD", "SUBSTITUTION_FORWARD", "F32",
     "F64", "PLAN_MIXED", "PLAN_VALIDATE", "STATUS", "EXPORTS", "LIB_PATH",
     "Config", "SptrsvError", "lib", "check", "DeviceArray", "Plan", "Graph",

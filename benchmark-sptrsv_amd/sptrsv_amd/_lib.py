@@ -23,6 +23,7 @@ PLAN_VALIDATE = 1
 PLAN_MIXED = 2
 ENGINE_GROUP = 0
 ENGINE_LANE = 1
+ENGINE_ROW = 2
 
 STATUS = {
     0: "OK",

@@ -188,7 +188,7 @@ typedef struct {
   int window;           /* LDS window, columns (max, for NNZ partition) */
   int partition;        /* SPTRSV_COL_PARTITION / SPTRSV_NNZ_PARTITION */
   int grid;             /* workgroups launched; 0 = resident capacity */
-  int engine;           /* SPTRSV_ENGINE_GROUP or SPTRSV_ENGINE_LANE */
+  int engine;           /* SPTRSV_ENGINE_GROUP, _LANE or _ROW */
 } sptrsv_config_t;
 
 /* Executor engines.  GROUP: lane groups of a wavefront per column, counters
@@ -202,6 +202,14 @@ typedef struct {
  * bytes). */
 #define SPTRSV_ENGINE_GROUP 0
 #define SPTRSV_ENGINE_LANE 1
+/* ROW (no counterpart in the reference, whose executor only pushes): pulls
+ * over the CSR of the factor, lanes_per_column (4, 16 or 64) lanes per row
+ * and right-hand side reading each entry's solution, which is its own
+ * arrival flag (x is filled with a sentinel first), so no atomics accumulate
+ * and a row fed by hundreds of columns costs no serialised updates.  For
+ * wide, shallow factors (random power-law rows); window and partition are
+ * unused and the grid never exceeds the resident capacity. */
+#define SPTRSV_ENGINE_ROW 2
 
 #define SPTRSV_PLAN_VALIDATE 1 /* check the factor at creation */
 #define SPTRSV_PLAN_MIXED 2    /* keep FP32 values + FP64 CSR for sptrsv_plan_solve_ir */
