@@ -385,6 +385,7 @@ struct WindowSet {
   DevBuf bstart;  // int[#batches]: batch starts, window-local, by window
   DevBuf boff;    // int[nwin+1]: each window's range in bstart
   int maxnb = 1;  // most batches in one window
+  bool has_wide = false;  // some batch is longer than a wavefront
   DevBuf roff;    // int[nwin+1]
   int nslots = -1;
 };
@@ -675,9 +676,15 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   if (n > 0) {
     DevBuf dl;
     ST_TRY(dl.ensure((size_t)n, false));
-    k_links<<<grid_for(n, 256), 256, 0, s>>>(
-        p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
-        dl.as<unsigned char>());
+    const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
+    if (p->vt == SPTRSV_F64)
+      k_links<double><<<grid_for(n, 256), 256, 0, s>>>(
+          p->colPtr, p->rowIdx, (const double *)p->val, p->inDegree.as<int>(),
+          n, bwd, dl.as<unsigned char>());
+    else
+      k_links<float><<<grid_for(n, 256), 256, 0, s>>>(
+          p->colPtr, p->rowIdx, (const float *)p->val, p->inDegree.as<int>(),
+          n, bwd, dl.as<unsigned char>());
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
       e = hipMemcpyAsync(link.data(), dl.p, (size_t)n, hipMemcpyDeviceToHost, s);
@@ -693,22 +700,36 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
     off[w] = (int)local.size();
     if (whi <= wlo) continue;
     long links = 0;
-    for (int q = wlo + 1; q < whi; q++) links += link[q];
+    for (int q = wlo + 1; q < whi; q++) links += link[q] != 0;
     const bool chained = 2 * links >= (long)(whi - wlo - 1) && whi - wlo > 1;
     int start = wlo;
+    bool pure = true;  // every link inside the current batch is pure
     local.push_back(0);
     first.push_back(wlo);
     for (int q = wlo; q < whi; q++) {
       const int len = q - start;
-      if (q > wlo && (len >= kWave || (chained && !link[q] && len >= kMinChainBatch))) {
+      // a run of pure links grows past a wavefront into a wide batch of up
+      // to kWave * kWideC columns
+      const bool wide_ok = pure && link[q] == 2 && len < kWave * kWideC;
+      if (q > wlo && ((len >= kWave && !wide_ok) ||
+                      (chained && !link[q] && len >= kMinChainBatch))) {
         start = q;
+        pure = true;
         local.push_back(q - wlo);
         first.push_back(q);
+      } else if (q > start) {
+        pure = pure && link[q] == 2;
       }
       owner[q] = (int)first.size() - 1;
     }
   }
   off[ws->nwin] = (int)local.size();
+  ws->has_wide = false;
+  for (int w = 0; w < ws->nwin; w++)
+    for (int i = off[w]; i < off[w + 1]; i++) {
+      const int end = i + 1 < off[w + 1] ? local[i + 1] : wp[w + 1] - wp[w];
+      if (end - local[i] > kWave) ws->has_wide = true;
+    }
   ws->maxnb = 1;
   for (int w = 0; w < ws->nwin; w++) ws->maxnb = std::max(ws->maxnb, off[w + 1] - off[w]);
   const size_t nb = std::max<size_t>(local.size(), 1);
@@ -844,12 +865,26 @@ using LaneKernel = void (*)(LaneArgs<T>);
 
 // Lane-engine kernel for R right-hand sides per lane (1, 2, 4 or 8).
 template <typename T>
-LaneKernel<T> pick_lane(bool bwd, int R) {
+LaneKernel<T> pick_lane_w(bool bwd, int R) {
   switch (R) {
-    case 1: return bwd ? k_exec_lane<T, true, 1> : k_exec_lane<T, false, 1>;
-    case 2: return bwd ? k_exec_lane<T, true, 2> : k_exec_lane<T, false, 2>;
-    case 4: return bwd ? k_exec_lane<T, true, 4> : k_exec_lane<T, false, 4>;
-    case 8: return bwd ? k_exec_lane<T, true, 8> : k_exec_lane<T, false, 8>;
+    case 1: return bwd ? k_exec_lane<T, true, 1, true> : k_exec_lane<T, false, 1, true>;
+    case 2: return bwd ? k_exec_lane<T, true, 2, true> : k_exec_lane<T, false, 2, true>;
+    case 4: return bwd ? k_exec_lane<T, true, 4, true> : k_exec_lane<T, false, 4, true>;
+    case 8: return bwd ? k_exec_lane<T, true, 8, true> : k_exec_lane<T, false, 8, true>;
+    default: return nullptr;
+  }
+}
+
+// The instantiation with wide batches only where the window set has one
+// (the extra path costs the others a few registers).
+template <typename T>
+LaneKernel<T> pick_lane(bool bwd, int R, bool wide) {
+  if (wide) return pick_lane_w<T>(bwd, R);
+  switch (R) {
+    case 1: return bwd ? k_exec_lane<T, true, 1, false> : k_exec_lane<T, false, 1, false>;
+    case 2: return bwd ? k_exec_lane<T, true, 2, false> : k_exec_lane<T, false, 2, false>;
+    case 4: return bwd ? k_exec_lane<T, true, 4, false> : k_exec_lane<T, false, 4, false>;
+    case 8: return bwd ? k_exec_lane<T, true, 8, false> : k_exec_lane<T, false, 8, false>;
     default: return nullptr;
   }
 }
@@ -973,7 +1008,8 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
                            slot_empty_host<T>(), s));
     }
     LaneKernel<T> kern =
-        pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L->R);
+        pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L->R,
+                     L->ws->has_wide);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     if (L->lds > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)kern,
@@ -1072,7 +1108,8 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.err = p->sync.as<int>() + 2;
     la.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
     LaneKernel<T> kern =
-        pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L.R);
+        pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L.R,
+                     L.ws->has_wide);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     kern<<<L.grid, c.block, L.lds, s>>>(la);
     HIP_TRY(hipGetLastError());

@@ -163,6 +163,169 @@ __host__ __device__ inline size_t lane_lds_bytes(size_t window, int R,
          sizeof(int) * (window / kMinChainBatch + 2);
 }
 
+// Most columns per lane in a wide batch: a run of pure links (see k_links)
+// longer than a wavefront is one batch of up to kWave * kWideC columns.
+constexpr int kWideC = 32;
+
+// r / d from r * (1 / d) and one correction step.
+template <typename T>
+__device__ __forceinline__ T wide_quot(T r, T d, T inv) {
+  const T qv = r * inv;
+  return __builtin_fma(__builtin_fma(-qv, d, r), inv, qv);
+}
+
+// The last column of a wide batch publishes right-hand side q of its x:
+// slots first, then LDS values (the counters follow a release).
+template <typename T, int R>
+__device__ __forceinline__ void wide_publish(const LaneArgs<T> &a, T *s_val,
+                                             T *slots, const int (&tg)[kPF],
+                                             const T (&vv)[kPF], int lo, int hi,
+                                             T x, int q) {
+#pragma unroll
+  for (int i = 0; i < kPF; i++) {
+    if (tg[i] >= 0)
+      xchg_agent(&slots[(size_t)tg[i] * R + q], x * vv[i]);
+    else if (tg[i] != kRegTarget)
+      add_wg(&s_val[(-tg[i] - 1) * R + q], -(x * vv[i]));
+  }
+  for (int j = lo + kPF; j < hi; j++) {
+    const int tj = a.etgt[j];
+    const T v = a.val[j];
+    if (tj >= 0)
+      xchg_agent(&slots[(size_t)tj * R + q], x * v);
+    else if (tj != kRegTarget)
+      add_wg(&s_val[(-tj - 1) * R + q], -(x * v));
+  }
+}
+
+// ... and then counts its LDS arrivals.
+template <typename T>
+__device__ __forceinline__ void wide_count(const LaneArgs<T> &a, int *s_cnt,
+                                           const int (&tg)[kPF], int lo, int hi) {
+#pragma unroll
+  for (int i = 0; i < kPF; i++)
+    if (tg[i] < 0 && tg[i] != kRegTarget) add_wg(&s_cnt[-tg[i] - 1], 1);
+  for (int j = lo + kPF; j < hi; j++) {
+    const int tj = a.etgt[j];
+    if (tj < 0 && tj != kRegTarget) add_wg(&s_cnt[-tj - 1], 1);
+  }
+}
+
+// Wide batch [base, bend) (window-local, more than 64 columns): every column
+// but the last has a single off-diagonal entry, into the next column, which
+// has no other input, so
+//     x_k = r_k / d_k + c_k * x_{k-1},   c_k = -L(k, k-1) / d_k,  |c_k| <= 1,
+// with r_k = b_k for every column but the first (whose LDS arrivals are the
+// batch's only wait).  Lane l holds C = ceil(len / 64) consecutive columns.
+// Before the wait each lane replaces its columns' b by b / d in LDS and
+// composes its maps, x_last = A + K * x_in (lane 0 leaves its first column
+// out: A = A_rest + P * a_0, P the product of its other coefficients); after
+// the wait lane 0 adds a_0, the wavefront scans the compositions with DPP,
+// the batch's last column (the last of lane `owner`) publishes at once, and
+// then each lane expands its columns from its predecessor lane's last x.
+// Returns false when the stall watch fired.
+template <typename T, bool BWD, int R>
+__device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
+                                          int *s_cnt, int base, int bend,
+                                          int wlo, int tile, int rv) {
+  const int lane = __lane_id();
+  const int n = a.n;
+  const int len = bend - base;
+  const int C = (len + kWave - 1) / kWave;
+  const int k0 = base + lane * C;
+  const int mine = max(0, min(C, bend - k0));
+  T cf[kWideC];                      // coefficient of the previous column's x
+  T d0 = T(1), iv0 = T(1);           // the batch's first column (lane 0)
+  T A[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) A[q] = T(0);
+  T K = T(1), P = T(1);
+#pragma unroll
+  for (int m = 0; m < kWideC; m++) {
+    cf[m] = T(0);
+    if (m < mine) {
+      const int p = wlo + k0 + m;
+      const int c = BWD ? n - 1 - p : p;
+      const T d = a.val[BWD ? a.colPtr[c + 1] - 1 : a.colPtr[c]];
+      const T inv = T(1) / d;
+      if (k0 + m > base) {  // the pure link from the column at p - 1
+        const int cp = BWD ? n - p : p - 1;
+        const T lv = a.val[BWD ? a.colPtr[cp + 1] - 2 : a.colPtr[cp] + 1];
+        cf[m] = -wide_quot(lv, d, inv);
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+          const T qv = wide_quot(s_val[(k0 + m) * R + q], d, inv);
+          s_val[(k0 + m) * R + q] = qv;
+          A[q] = __builtin_fma(cf[m], A[q], qv);
+        }
+      } else {
+        d0 = d;
+        iv0 = inv;
+      }
+      K = cf[m] * K;
+      if (m > 0) P = cf[m] * P;
+    }
+  }
+  // the last column's entries (they leave the batch), first kPF in registers
+  const int kl = bend - 1;
+  const int owner = (len - 1) / C;
+  const int pl = wlo + kl;
+  const int cl = BWD ? n - 1 - pl : pl;
+  const int fl = a.colPtr[cl], ll = a.colPtr[cl + 1];
+  const int lo = BWD ? fl : fl + 1, hi = BWD ? ll - 1 : ll;
+  int tg[kPF];
+  T vv[kPF];
+#pragma unroll
+  for (int i = 0; i < kPF; i++) {
+    const bool has = lane == owner && lo + i < hi;
+    tg[i] = has ? a.etgt[lo + i] : kRegTarget;
+    vv[i] = has ? a.val[lo + i] : T(0);
+  }
+  // the first column's arrivals (its predecessor is in an earlier batch)
+  const int pf = wlo + base;
+  const int2 nd = a.need[BWD ? n - 1 - pf : pf];
+  const int need0 = nd.x + nd.y * rv;
+  T *slots = a.slots + (size_t)tile * a.nslots * R;
+  StallWatch watch;
+  while (ld_wg(&s_cnt[base]) != need0)
+    if (watch.step(false, a.err, a.wait_cap)) return false;
+  __builtin_amdgcn_s_setprio(3);
+  fence_acq_wg();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const T a0 = wide_quot(s_val[base * R + q], d0, iv0);
+      s_val[base * R + q] = a0;
+      A[q] = __builtin_fma(P, a0, A[q]);
+    }
+  }
+  chain_scan<T, R>(A, K);  // lane 0's K is 0: every A is now absolute
+  // the batch's last column is the owner's last: publish it at once; the
+  // other columns' x are only read at window end
+  if (lane == owner)
+    for (int q = 0; q < rv; q++)
+      wide_publish<T, R>(a, s_val, slots, tg, vv, lo, hi, A[q], q);
+  fence_rel_wg();
+  if (lane == owner) wide_count(a, s_cnt, tg, lo, hi);
+  __builtin_amdgcn_s_setprio(0);
+  T x[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    x[q] = __shfl(A[q], lane > 0 ? lane - 1 : 0);
+    if (lane == 0) x[q] = T(0);
+  }
+#pragma unroll
+  for (int m = 0; m < kWideC; m++)
+    if (m < mine) {
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        x[q] = __builtin_fma(cf[m], x[q], s_val[(k0 + m) * R + q]);
+        s_val[(k0 + m) * R + q] = x[q];
+      }
+    }
+  return true;
+}
+
 // Compute wavefront cw handles batches of up to 64 consecutive columns (lane
 // = column) cw, cw + ncw, ..., each column for the R right-hand sides of its
 // tile (rv of them valid).  Each row r may have one in-batch predecessor p(r)
@@ -180,7 +343,7 @@ __host__ __device__ inline size_t lane_lds_bytes(size_t window, int R,
 // whose chain is not complete wait for the next round; the lowest unsolved
 // lane always completes once its inputs from earlier batches arrive, so the
 // batch always drains.
-template <typename T, bool BWD, int R>
+template <typename T, bool BWD, int R, bool WIDE>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, const int *s_bst,
                                             int nb, int wlo, int wlen,
@@ -195,6 +358,13 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     // bounds staged in LDS
     const int base = s_bst[bi];
     const int bend = bi + 1 < nb ? s_bst[bi + 1] : wlen;
+    if constexpr (WIDE) {
+      if (bend - base > kWave) {  // a run of pure links: one scan
+        if (!lane_wide<T, BWD, R>(a, s_val, s_cnt, base, bend, wlo, tile, rv))
+          stopped = true;
+        continue;
+      }
+    }
     const int k = base + lane;
     bool active = k < bend;
     int lo = 0, hi = 0, need_lds = 0, ne = 0, pl = lane;
@@ -474,7 +644,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   }
 }
 
-template <typename T, bool BWD, int R>
+template <typename T, bool BWD, int R, bool WIDE>
 __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
@@ -513,7 +683,7 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
     if (wave < a.nrecv)
       lane_receive<T, R>(a, s_val, s_cnt, win, tile, rv, wave);
     else
-      lane_compute<T, BWD, R>(a, s_val, s_cnt, s_bst, nb, wlo, wlen, tile,
+      lane_compute<T, BWD, R, WIDE>(a, s_val, s_cnt, s_bst, nb, wlo, wlen, tile,
                               rv, wave - a.nrecv, ncw);
     __syncthreads();
     for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
@@ -561,9 +731,16 @@ __global__ void k_lane_targets(const int *__restrict__ colPtr,
 
 // Chain links by solve position: link[p] = 1 when the column at position
 // p - 1 has an entry in the row at position p (the nearest off-diagonal
-// entry: first after the diagonal forward, last before it backward).
+// entry: first after the diagonal forward, last before it backward); 2 when
+// that link is pure: the column at p - 1 has no other off-diagonal entry,
+// the row at p no other input (inDeg counts the diagonal) and
+// |L(p, p-1)| <= |L(p, p)|.  A run of pure links is a first-order
+// recurrence that a wide batch solves with one wavefront scan.
+template <typename T>
 __global__ void k_links(const int *__restrict__ colPtr,
-                        const int *__restrict__ rowIdx, int n, int backward,
+                        const int *__restrict__ rowIdx,
+                        const T *__restrict__ val,
+                        const int *__restrict__ inDeg, int n, int backward,
                         unsigned char *link) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += gridDim.x * blockDim.x) {
@@ -572,10 +749,12 @@ __global__ void k_links(const int *__restrict__ colPtr,
       const int c = backward ? n - p : p - 1;       // column at position p - 1
       const int want = backward ? n - 1 - p : p;    // row at position p
       const int f = colPtr[c], l = colPtr[c + 1];
-      if (backward)
-        v = l - 2 >= f && rowIdx[l - 2] == want;
-      else
-        v = f + 1 < l && rowIdx[f + 1] == want;
+      const int j = backward ? l - 2 : f + 1;       // the nearest entry
+      if (j >= f && j < l && (backward ? j < l - 1 : j > f) && rowIdx[j] == want) {
+        v = 1;
+        const T d = val[backward ? colPtr[want + 1] - 1 : colPtr[want]];
+        if (l - f == 2 && inDeg[want] == 2 && fabs(val[j]) <= fabs(d)) v = 2;
+      }
     }
     link[p] = v;
   }

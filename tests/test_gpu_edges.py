@@ -110,3 +110,79 @@ def test_graph_replays_multi_rhs_lane():
         stream.synchronize()
         assert plan.check(stream) == 0
         assert_close(dx.to_host(), ref, 1e-12)
+
+
+# ------------------------------------------------------------ wide batches
+# A run of pure links (each column's only off-diagonal entry feeds the next
+# column, which has no other input) longer than 64 columns is one wide batch
+# of up to 512 columns, 8 per lane, solved by one wavefront scan.
+def _chain_with_breaks(n, every, seed):
+    """Bidiagonal chain plus an extra entry every `every` columns, which
+    breaks the run of pure links (the extra entry's column has two
+    off-diagonal entries and its target row two inputs)."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(n, dtype=np.int64)
+    extra_c = np.arange(0, n - 40, every, dtype=np.int64)
+    c = np.concatenate([i, i[:-1], extra_c])
+    r = np.concatenate([i, i[:-1] + 1, extra_c + 37])
+    v = np.concatenate([rng.uniform(2.0, 3.0, n), rng.uniform(-1.0, 1.0, n - 1),
+                        rng.uniform(-0.5, 0.5, len(extra_c))])
+    p, ri, vv = mx._from_columns(n, c, r, v)
+    return mx.Factor(p, ri, vv, name=f"chain{n}_e{every}")
+
+
+@pytest.mark.parametrize("window", [256, 1000, 8192])
+@pytest.mark.parametrize("n", [65, 100, 511, 512, 513, 1500, 20_000])
+def test_wide_batches_bidiagonal(n, window):
+    f = mx.bidiagonal(n)
+    b = f.spmv(mx.x_uniform(n, seed=n))
+    x, _ = plan_solve(f, b, cfg=dict(engine=LANE, block=512, window=window))
+    assert_close(x, oracle_x(f, b), 1e-12)
+
+
+def test_wide_batches_known_answer():
+    f = mx.bidiagonal(100_000)  # diag 2, sub -1: x = 1 stays dyadic
+    b = f.spmv(np.ones(f.n))
+    x, _ = plan_solve(f, b, cfg=dict(engine=LANE, block=1024, window=8192))
+    assert_close(x, np.ones(f.n), 1e-14)
+
+
+@pytest.mark.parametrize("every", [50, 300, 2000])
+@pytest.mark.parametrize("backward", [0, 1])
+def test_wide_batches_broken_chains(every, backward):
+    f = _chain_with_breaks(30_000, every, seed=every)
+    if backward:
+        f = mx.upper_from_lower(f)
+    b = f.spmv(mx.x_uniform(f.n, seed=2))
+    for cfg in (dict(engine=LANE, block=1024, window=8192), dict(engine=LANE, block=256, window=700)):
+        x, _ = plan_solve(f, b, subst=backward, cfg=cfg)
+        assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
+
+
+@pytest.mark.parametrize("R,rhs", [(1, 3), (2, 3), (4, 4), (8, 5)])
+def test_wide_batches_multi_rhs(R, rhs):
+    f = _chain_with_breaks(12_000, 900, seed=R)
+    X = mx.x_normal(f.n, rhs, seed=R)
+    B = f.spmv(X)
+    window = 1024 if R == 8 else 2048  # LDS: (window + 64) * (8 R + 4) bytes
+    x, _ = plan_solve(f, B, rhs=rhs, cfg=dict(engine=LANE, rhs_per_lane=R, block=512, window=window))
+    assert_close(x, oracle_x(f, B, rhs=rhs), 1e-12)
+
+
+def test_wide_batches_fp32_and_large_links():
+    # every 7th link has |L(c+1, c)| > |L(c+1, c+1)|, which makes it impure:
+    # the runs between those links are shorter than a wide batch needs only
+    # where they are; FP32 plan on a pure chain
+    n = 5000
+    i = np.arange(n, dtype=np.int64)
+    sub = np.where(i[:-1] % 7 == 6, -2.5, -1.0)
+    p, ri, vv = mx._from_columns(n, np.concatenate([i, i[:-1]]), np.concatenate([i, i[:-1] + 1]),
+                                 np.concatenate([np.full(n, 2.0), sub]))
+    f = mx.Factor(p, ri, vv, name="chain_big_links")
+    b = f.spmv(mx.x_uniform(n, seed=1))
+    x, _ = plan_solve(f, b, cfg=dict(engine=LANE, block=512, window=8192))
+    assert_close(x, oracle_x(f, b), 1e-12)
+    f = mx.bidiagonal(n)
+    b = f.spmv(mx.x_uniform(n, seed=2))
+    x, _ = plan_solve(f, b, dtype=np.float32, cfg=dict(engine=LANE, block=512, window=8192))
+    assert_close(x, oracle_x(f, b, dtype=np.float32), 2e-5)
