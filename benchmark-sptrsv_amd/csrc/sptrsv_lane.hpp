@@ -353,6 +353,8 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   const int n = a.n;
   T *slots = a.slots + (size_t)tile * a.nslots * R;
   bool stopped = false;  // the stall watch fired: leave the window
+  int pf_bi = -1, pf_f = 0, pf_l = 0;
+  int2 pf_nd = make_int2(0, 0), pf_ch = make_int2(-1, 0);
   for (int bi = cw; bi < nb && !stopped; bi += ncw) {
     // a batch: up to 64 consecutive columns (window-local [base, bend)),
     // bounds staged in LDS
@@ -374,14 +376,15 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     if (active) {
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
-      const int f = a.colPtr[c], l = a.colPtr[c + 1];
+      int f, l;
+      int2 nd, ch;
+      if (pf_bi == bi) { f = pf_f; l = pf_l; nd = pf_nd; ch = pf_ch; }
+      else { f = a.colPtr[c]; l = a.colPtr[c + 1]; nd = a.need[c]; ch = a.chain[c]; }
       diag = a.val[BWD ? l - 1 : f];
       inv = T(1) / diag;  // off the critical path
       lo = BWD ? f : f + 1;
       hi = BWD ? l - 1 : l;
       ne = min(hi - lo, kPF);
-      const int2 nd = a.need[c];
-      const int2 ch = a.chain[c];
       // receivers count one arrival per remote entry and right-hand side
       need_lds = nd.x + nd.y * rv;
       if (ch.x >= 0 && ch.x < lane) {
@@ -397,6 +400,24 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         const int j = lo + i;
         tg[i] = j < hi ? a.etgt[j] : kRegTarget;
         vv[i] = j < hi ? a.val[j] : T(0);
+      }
+    }
+    {  // the next batch's first-level metadata, in flight during this one
+      const int nbi = bi + ncw;
+      pf_bi = -1;
+      if (nbi < nb) {
+        const int nbase = s_bst[nbi];
+        const int nbend = nbi + 1 < nb ? s_bst[nbi + 1] : wlen;
+        const int kk = nbase + lane;
+        pf_bi = nbi;
+        if (kk < nbend && nbend - nbase <= kWave) {
+          const int pp = wlo + kk;
+          const int cc = BWD ? n - 1 - pp : pp;
+          pf_f = a.colPtr[cc];
+          pf_l = a.colPtr[cc + 1];
+          pf_nd = a.need[cc];
+          pf_ch = a.chain[cc];
+        }
       }
     }
     // LDS target of each register entry; entries that do not go to LDS add
