@@ -69,6 +69,9 @@ struct LaneArgs {
   const int4 *__restrict__ rlist; // columns with remote inputs, window order:
                                   // {local index, first slot, slot count, -}
   const int *__restrict__ roff;   // [nwin + 1] each window's rlist range
+  const int *__restrict__ worder; // claim order of the windows, or null
+                                  // (in order); a window's predecessors
+                                  // always come earlier
   T *slots;                       // [ntiles][nslots][R]
   const T *__restrict__ b;        // [n][rhs]
   T *__restrict__ x;              // [n][rhs]
@@ -685,8 +688,9 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
     __syncthreads();
     const int t = s_ticket;
     if (t >= total) break;
-    const int win = t / a.ntiles;
-    const int tile = t - win * a.ntiles;
+    const int wi = t / a.ntiles;
+    const int tile = t - wi * a.ntiles;
+    const int win = a.worder ? a.worder[wi] : wi;
     const int rv = min(R, a.rhs - tile * R);  // valid right-hand sides
     const int wlo = a.winPtr[win];
     const int wlen = a.winPtr[win + 1] - wlo;
