@@ -384,6 +384,7 @@ struct WindowSet {
   DevBuf chain;   // int2[n]: in-batch predecessor delivered by register
   DevBuf bstart;  // int[#batches]: batch starts, window-local, by window
   DevBuf boff;    // int[nwin+1]: each window's range in bstart
+  DevBuf bord;    // int[#batches]: by window, its batches in processing order
   int maxnb = 1;  // most batches in one window
   DevBuf worder;  // int[nwin]: claim order of the windows (batches of systems)
   bool has_order = false;
@@ -773,6 +774,52 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(bid->p, owner.data(), sizeof(int) * owner.size(),
                          hipMemcpyHostToDevice, s));
+  // processing order: each window's batches by (level in the window's batch
+  // DAG, index).  Every batch's in-window predecessors have lower levels, so
+  // the wavefront owning the lowest unfinished batch of that order has
+  // finished its earlier ones and that batch's inputs are complete: the
+  // round-robin deal of this order cannot deadlock, and wavefronts take
+  // batches in the order they become ready (anti-diagonals of a grid)
+  std::vector<int> lev(nb, 0);
+  if (n > 0 && !local.empty()) {
+    std::vector<int> bw(local.size());
+    for (int w = 0; w < ws->nwin; w++)
+      for (int i = off[w]; i < off[w + 1]; i++) bw[i] = w;
+    DevBuf dbw, dlev, dchg;
+    ST_TRY(dbw.ensure(sizeof(int) * bw.size(), false));
+    ST_TRY(dlev.ensure(sizeof(int) * bw.size(), true));
+    ST_TRY(dchg.ensure(sizeof(int), true));
+    HIP_TRY(hipMemcpyAsync(dbw.p, bw.data(), sizeof(int) * bw.size(),
+                           hipMemcpyHostToDevice, s));
+    const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
+    for (int pass = 0; pass <= ws->maxnb; pass++) {
+      int chg = 0;
+      HIP_TRY(hipMemsetAsync(dchg.p, 0, sizeof(int), s));
+      k_batch_level<<<grid_for(n, 256), 256, 0, s>>>(
+          p->colPtr, p->rowIdx, n, bwd, bid->as<int>(), dbw.as<int>(),
+          dlev.as<int>(), dchg.as<int>());
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(&chg, dchg.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (!chg) break;
+    }
+    HIP_TRY(hipMemcpyAsync(lev.data(), dlev.p, sizeof(int) * bw.size(),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  std::vector<int> order(nb, 0);
+  for (int w = 0; w < ws->nwin; w++) {
+    const int b0 = off[w], b1 = off[w + 1];
+    std::vector<int> idx(b1 - b0);
+    for (int i = 0; i < b1 - b0; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+      return lev[b0 + x] < lev[b0 + y];
+    });
+    for (int i = 0; i < b1 - b0; i++) order[b0 + i] = idx[i];
+  }
+  ST_TRY(ws->bord.ensure(sizeof(int) * nb, false));
+  HIP_TRY(hipMemcpyAsync(ws->bord.p, order.data(), sizeof(int) * nb,
+                         hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
   return SPTRSV_OK;
 }
@@ -1021,7 +1068,7 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     L->ntiles = (rhs + L->R - 1) / L->R;
     // window rows plus one dummy row per lane, then the batch starts
     L->lds = ((size_t)L->ws->maxw + 64) * ((size_t)L->R * sizeof(T) + sizeof(int)) +
-             sizeof(int) * (size_t)L->ws->maxnb;
+             2 * sizeof(int) * (size_t)L->ws->maxnb;
     if (L->lds > lane_lds_bytes(L->ws->maxw, L->R, sizeof(T)))
       return SPTRSV_ERR_INVALID_ARG;  // cannot happen: maxnb <= maxw/16 + 1
     DevBuf &sb = slot_buf<T>(p);
@@ -1117,6 +1164,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.winPtr = L.ws->winPtr.as<int>();
     la.bstart = L.ws->bstart.as<int>();
     la.boff = L.ws->boff.as<int>();
+    la.bord = L.ws->bord.as<int>();
     la.maxnb = L.ws->maxnb;
     la.rlist = L.ws->rlist.as<int4>();
     la.roff = L.ws->roff.as<int>();
@@ -1727,6 +1775,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     kv.second->chain.release();
     kv.second->bstart.release();
     kv.second->boff.release();
+    kv.second->bord.release();
     kv.second->worder.release();
     delete kv.second;
   }

@@ -66,6 +66,8 @@ struct LaneArgs {
   const int *__restrict__ winPtr; // [nwin + 1] window bounds, solve order
   const int *__restrict__ bstart; // batch starts, window-local, by window
   const int *__restrict__ boff;   // [nwin + 1] each window's bstart range
+  const int *__restrict__ bord;   // by window: its batches in processing
+                                  // order (level in the batch DAG, index)
   const int4 *__restrict__ rlist; // columns with remote inputs, window order:
                                   // {local index, first slot, slot count, -}
   const int *__restrict__ roff;   // [nwin + 1] each window's rlist range
@@ -159,11 +161,12 @@ constexpr int kRegTarget = -2147483647 - 1;
 constexpr int kMinChainBatch = 16;
 
 // LDS of a lane-engine workgroup: window rows plus one dummy row per lane
-// (R values and a counter each), then the window's batch starts.
+// (R values and a counter each), then the window's batch starts and its
+// batch processing order.
 __host__ __device__ inline size_t lane_lds_bytes(size_t window, int R,
                                                  size_t tsize) {
   return (window + kWave) * ((size_t)R * tsize + sizeof(int)) +
-         sizeof(int) * (window / kMinChainBatch + 2);
+         2 * sizeof(int) * (window / kMinChainBatch + 2);
 }
 
 // Most columns per lane in a wide batch: a run of pure links (see k_links)
@@ -349,8 +352,8 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
 template <typename T, bool BWD, int R, bool WIDE>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, const int *s_bst,
-                                            int nb, int wlo, int wlen,
-                                            int tile, int rv, int cw,
+                                            const int *s_bord, int nb, int wlo,
+                                            int wlen, int tile, int rv, int cw,
                                             int ncw) {
   const int lane = __lane_id();
   const int n = a.n;
@@ -358,9 +361,12 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   bool stopped = false;  // the stall watch fired: leave the window
   int pf_bi = -1, pf_f = 0, pf_l = 0;
   int2 pf_nd = make_int2(0, 0), pf_ch = make_int2(-1, 0);
-  for (int bi = cw; bi < nb && !stopped; bi += ncw) {
+  // the window's batches in level order, dealt round-robin: wavefront cw
+  // takes positions cw, cw + ncw, ... of that order
+  for (int pos = cw; pos < nb && !stopped; pos += ncw) {
     // a batch: up to 64 consecutive columns (window-local [base, bend)),
     // bounds staged in LDS
+    const int bi = s_bord[pos];
     const int base = s_bst[bi];
     const int bend = bi + 1 < nb ? s_bst[bi + 1] : wlen;
     if constexpr (WIDE) {
@@ -406,9 +412,9 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       }
     }
     {  // the next batch's first-level metadata, in flight during this one
-      const int nbi = bi + ncw;
       pf_bi = -1;
-      if (nbi < nb) {
+      if (pos + ncw < nb) {
+        const int nbi = s_bord[pos + ncw];
         const int nbase = s_bst[nbi];
         const int nbend = nbi + 1 < nb ? s_bst[nbi + 1] : wlen;
         const int kk = nbase + lane;
@@ -676,6 +682,7 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
   // publishing, then the window's batch starts
   int *s_cnt = reinterpret_cast<int *>(s_val + (size_t)(a.maxw + kWave) * R);
   int *s_bst = s_cnt + a.maxw + kWave;
+  int *s_bord = s_bst + a.maxnb;
   __shared__ int s_ticket;
   const int total = a.nwin * a.ntiles;
   const int wave = threadIdx.x / kWave;
@@ -703,13 +710,16 @@ __global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
     }
     for (int k = threadIdx.x; k < wlen; k += blockDim.x) s_cnt[k] = 0;
     const int b0 = a.boff[win], nb = a.boff[win + 1] - b0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s_bst[i] = a.bstart[b0 + i];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      s_bst[i] = a.bstart[b0 + i];
+      s_bord[i] = a.bord[b0 + i];
+    }
     __syncthreads();
     if (wave < a.nrecv)
       lane_receive<T, R>(a, s_val, s_cnt, win, tile, rv, wave);
     else
-      lane_compute<T, BWD, R, WIDE>(a, s_val, s_cnt, s_bst, nb, wlo, wlen, tile,
-                              rv, wave - a.nrecv, ncw);
+      lane_compute<T, BWD, R, WIDE>(a, s_val, s_cnt, s_bst, s_bord, nb, wlo,
+                                    wlen, tile, rv, wave - a.nrecv, ncw);
     __syncthreads();
     for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
       const int k = R == 1 ? i : i / R;
@@ -831,6 +841,32 @@ __global__ void k_chain_mark(const unsigned long long *__restrict__ key, int n,
     const int j = (int)(unsigned)(kk & 0xffffffffull);
     chain[r] = make_int2(pc - bfirst[bid[pc]], j);  // lane in its batch
     etgt[j] = kRegTarget;
+  }
+}
+
+// One relaxation pass of the batch levels: for every entry (r, c) whose
+// columns lie in different batches of one window, lev[batch(r)] >=
+// lev[batch(c)] + 1 (in-batch links weigh nothing; cross-window inputs are
+// not counted).  Repeated until no level changes; levels only grow and are
+// bounded by the batches of a window, so the passes end.
+__global__ void k_batch_level(const int *__restrict__ colPtr,
+                              const int *__restrict__ rowIdx, int n,
+                              int backward, const int *__restrict__ bid,
+                              const int *__restrict__ bwin, int *lev,
+                              int *changed) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += gridDim.x * blockDim.x) {
+    const int p = backward ? n - 1 - c : c;
+    const int bc = bid[p];
+    const int lc = ld_agent(&lev[bc]) + 1;
+    const int f = colPtr[c], l = colPtr[c + 1];
+    const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
+    for (int j = lo; j < hi; j++) {
+      const int r = rowIdx[j];
+      const int br = bid[backward ? n - 1 - r : r];
+      if (br != bc && bwin[br] == bwin[bc] && atomicMax(&lev[br], lc) < lc)
+        atomicOr(changed, 1);
+    }
   }
 }
 
