@@ -781,29 +781,15 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   // round-robin deal of this order cannot deadlock, and wavefronts take
   // batches in the order they become ready (anti-diagonals of a grid)
   std::vector<int> lev(nb, 0);
-  if (n > 0 && !local.empty()) {
-    std::vector<int> bw(local.size());
-    for (int w = 0; w < ws->nwin; w++)
-      for (int i = off[w]; i < off[w + 1]; i++) bw[i] = w;
-    DevBuf dbw, dlev, dchg;
-    ST_TRY(dbw.ensure(sizeof(int) * bw.size(), false));
-    ST_TRY(dlev.ensure(sizeof(int) * bw.size(), true));
-    ST_TRY(dchg.ensure(sizeof(int), true));
-    HIP_TRY(hipMemcpyAsync(dbw.p, bw.data(), sizeof(int) * bw.size(),
-                           hipMemcpyHostToDevice, s));
-    const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
-    for (int pass = 0; pass <= ws->maxnb; pass++) {
-      int chg = 0;
-      HIP_TRY(hipMemsetAsync(dchg.p, 0, sizeof(int), s));
-      k_batch_level<<<grid_for(n, 256), 256, 0, s>>>(
-          p->colPtr, p->rowIdx, n, bwd, bid->as<int>(), dbw.as<int>(),
-          dlev.as<int>(), dchg.as<int>());
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(&chg, dchg.p, sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      if (!chg) break;
-    }
-    HIP_TRY(hipMemcpyAsync(lev.data(), dlev.p, sizeof(int) * bw.size(),
+  if (n > 0 && !local.empty() && ws->nwin > 0) {
+    DevBuf dlev;
+    ST_TRY(dlev.ensure(sizeof(int) * local.size(), true));
+    k_batch_level<<<ws->nwin, kWave, 0, s>>>(
+        p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
+        bfirst->as<int>(), ws->boff.as<int>(), ws->winPtr.as<int>(),
+        bid->as<int>(), dlev.as<int>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(lev.data(), dlev.p, sizeof(int) * local.size(),
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
   }

@@ -844,28 +844,39 @@ __global__ void k_chain_mark(const unsigned long long *__restrict__ key, int n,
   }
 }
 
-// One relaxation pass of the batch levels: for every entry (r, c) whose
-// columns lie in different batches of one window, lev[batch(r)] >=
-// lev[batch(c)] + 1 (in-batch links weigh nothing; cross-window inputs are
-// not counted).  Repeated until no level changes; levels only grow and are
-// bounded by the batches of a window, so the passes end.
+// Batch levels of one window per workgroup of one wavefront: the batches
+// are visited in index order, which is topological (every entry goes from a
+// column to a later row), so a batch's level is final when it is visited;
+// its columns then push level + 1 to the batches of their rows in the same
+// window (in-batch links weigh nothing; cross-window inputs are not
+// counted).
 __global__ void k_batch_level(const int *__restrict__ colPtr,
                               const int *__restrict__ rowIdx, int n,
-                              int backward, const int *__restrict__ bid,
-                              const int *__restrict__ bwin, int *lev,
-                              int *changed) {
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += gridDim.x * blockDim.x) {
-    const int p = backward ? n - 1 - c : c;
-    const int bc = bid[p];
-    const int lc = ld_agent(&lev[bc]) + 1;
-    const int f = colPtr[c], l = colPtr[c + 1];
-    const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
-    for (int j = lo; j < hi; j++) {
-      const int r = rowIdx[j];
-      const int br = bid[backward ? n - 1 - r : r];
-      if (br != bc && bwin[br] == bwin[bc] && atomicMax(&lev[br], lc) < lc)
-        atomicOr(changed, 1);
+                              int backward, const int *__restrict__ bfirst,
+                              const int *__restrict__ boff,
+                              const int *__restrict__ winPtr,
+                              const int *__restrict__ bid, int *lev) {
+  const int w = blockIdx.x;
+  const int b0 = boff[w], b1 = boff[w + 1];
+  const int wend = winPtr[w + 1];
+  for (int bi = b0; bi < b1; bi++) {
+    // the pushes of earlier batches are complete and visible (atomics at the
+    // coherence point, read with an agent-scope load)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const int lb = ld_agent(&lev[bi]) + 1;
+    const int ps = bfirst[bi], pe = bi + 1 < b1 ? bfirst[bi + 1] : wend;
+    for (int q = ps + (int)threadIdx.x; q < pe; q += blockDim.x) {
+      const int c = backward ? n - 1 - q : q;
+      const int f = colPtr[c], l = colPtr[c + 1];
+      const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
+      for (int j = lo; j < hi; j++) {
+        const int r = rowIdx[j];
+        const int pr = backward ? n - 1 - r : r;
+        if (pr < wend) {
+          const int br = bid[pr];
+          if (br != bi) atomicMax(&lev[br], lb);
+        }
+      }
     }
   }
 }
