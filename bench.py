@@ -196,6 +196,9 @@ def main():
         if bwd:
             from sptrsv_amd import matrices as mx
             f = mx.upper_from_lower(f)
+    if levels is None:
+        # DAG depth of the factor (findlevel.h's definition), host analysis
+        levels = S.levels(f.colPtr, f.rowIdx)[0]
     rhs = args.rhs
     if rhs > 1:
         from sptrsv_amd import matrices as mx
