@@ -151,8 +151,16 @@ __global__ void __launch_bounds__(1024) k_exec_row(RowArgs<T> a) {
           pub = true;
         }
       }
+      // a wavefront per row: a short fixed pause (its inputs are usually one
+      // publish away; 8% faster on the 10^6 power-law factor); narrower
+      // groups, many rows per wavefront: exponential back-off (a fixed pause
+      // made the 700^2 grid 30% slower)
       const bool any = any_done || __ballot(got != 0) != 0;
-      backoff(level, any);
+      if constexpr (G == kWave) {
+        if (!any) __builtin_amdgcn_s_sleep(1);
+      } else {
+        backoff(level, any);
+      }
       if (watch.step(any, a.err, a.wait_cap)) {
         stopped = true;
         break;
