@@ -186,3 +186,34 @@ def test_wide_batches_fp32_and_large_links():
     b = f.spmv(mx.x_uniform(n, seed=2))
     x, _ = plan_solve(f, b, dtype=np.float32, cfg=dict(engine=LANE, block=512, window=8192))
     assert_close(x, oracle_x(f, b, dtype=np.float32), 2e-5)
+
+
+# ------------------------------------------------------ batch deal and order
+# Grid rows of 2..11 batches: the level-ordered deal of a window's batches to
+# its wavefronts (anti-diagonals) instead of index round-robin.
+@pytest.mark.parametrize("nx", [130, 200, 700, 769])
+@pytest.mark.parametrize("backward", [0, 1])
+def test_level_ordered_deal_on_wide_grids(nx, backward):
+    f = mx.laplacian2d_lower(nx, 40)
+    if backward:
+        f = mx.upper_from_lower(f)
+    b = f.spmv(mx.x_uniform(f.n, seed=nx))
+    for cfg in (dict(engine=LANE, block=1024, window=8192), dict(engine=LANE, block=256, window=1024)):
+        x, _ = plan_solve(f, b, subst=backward, cfg=cfg)
+        assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
+
+
+@pytest.mark.parametrize("backward", [0, 1])
+def test_batch_claim_order_largest_first(backward):
+    # systems of very different sizes in no order: windows are claimed
+    # largest system first, each system's windows in order
+    f = mx.laplacian2d_batch(40, lo=8, hi=200, seed=9)
+    if backward:
+        sys_ptr = f.sys_ptr  # U = L^T keeps the system boundaries
+        f = mx.upper_from_lower(f)
+        f.sys_ptr = sys_ptr
+    X = mx.x_uniform(f.n, seed=10)
+    b = f.spmv(X)
+    for cfg in (None, dict(engine=LANE, block=512, window=2048)):
+        x, _ = plan_solve(f, b, subst=backward, cfg=cfg)
+        assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
