@@ -417,9 +417,31 @@ struct sptrsv_plan_s {
   std::vector<int> sys_ptr;
   double avg_offdiag = 0.0;
   double far_frac = 0.0;  // off-diagonal entries >= kFarDistance from the diagonal
+  int period = 0;         // most common column reach (grid plane/row), or 0
 };
 
 namespace {
+
+// Lane-engine window for R right-hand sides: aligned to the factor's period
+// (whole grid rows or planes per window, so that no window holds the tail of
+// one plane and the head of the next: such a window waits on both at once,
+// and a 100^3 grid took 5.8 ms with 8192-column windows against 0.32 ms with
+// plane-aligned ones), otherwise the widest power of two up to 8192 that
+// fits in LDS.  Batches of systems keep the power of two (their windows
+// follow the systems).
+int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
+  int pow2 = 8192;
+  while (pow2 > 256 && lane_lds_bytes(pow2, R, tsize) > p->lds_max) pow2 /= 2;
+  const int D = p->period;
+  if (D <= 1 || p->nsys > 1) return pow2;
+  int wmax = 16384;  // the widest window that fits, a multiple of 64
+  while (wmax > 256 && lane_lds_bytes(wmax, R, tsize) > p->lds_max) wmax -= 64;
+  if (D <= pow2) return D * (pow2 / D);
+  if (D <= wmax) return D;
+  for (int k = 2; k <= 64; k++)
+    if (D % k == 0 && D / k <= wmax) return D / k;
+  return pow2;
+}
 
 sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   sptrsv_config_t c;
@@ -443,9 +465,7 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   c.rhs_lanes = 1;
   c.rhs_per_lane = rhs >= 4 ? 4 : rhs >= 2 ? 2 : 1;
   c.block = 1024;
-  c.window = 8192;
-  while (c.window > 256 && lane_lds_bytes(c.window, c.rhs_per_lane, 8) > p->lds_max)
-    c.window /= 2;
+  c.window = lane_window(p, c.rhs_per_lane, p->vt == SPTRSV_F64 ? 8 : 4);
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
   return c;
@@ -1307,6 +1327,40 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
                                                      p->inDegree.as<int>());
       if (hipGetLastError() != hipSuccess) st = SPTRSV_ERR_HIP;
     }
+    // the factor's period: the most common reach of a sample of columns
+    if (st == SPTRSV_OK && nnz > n) {
+      const int nsamp = std::min(n, 4096);
+      const int stride = std::max(1, n / nsamp);
+      DevBuf dr;
+      std::vector<int> reach(nsamp, 0);
+      st = dr.ensure(sizeof(int) * nsamp, false);
+      if (st == SPTRSV_OK) {
+        k_sample_reach<<<grid_for(nsamp, 256), 256, 0, s>>>(colPtr, rowIdx, n,
+                                                            stride, nsamp,
+                                                            dr.as<int>());
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(reach.data(), dr.p, sizeof(int) * nsamp,
+                           hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+          st = SPTRSV_ERR_HIP;
+      }
+      dr.release();
+      if (st == SPTRSV_OK) {
+        std::sort(reach.begin(), reach.end());
+        int best = 0, best_n = 0;
+        for (int i = 0; i < nsamp;) {
+          int j = i;
+          while (j < nsamp && reach[j] == reach[i]) j++;
+          if (reach[i] > 1 && j - i > best_n) {
+            best = reach[i];
+            best_n = j - i;
+          }
+          i = j;
+        }
+        // a period when it is the reach of at least 40% of the columns
+        p->period = 5 * best_n >= 2 * nsamp ? best : 0;
+      }
+    }
     // entries farther from the diagonal than the default lane window
     if (st == SPTRSV_OK && nnz > n) st = p->farcnt.ensure(sizeof(unsigned long long), true);
     if (st == SPTRSV_OK && nnz > n) {
@@ -1414,11 +1468,16 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
   };
   for (int R : {1, 2, 4, 8}) {
     if (R > 1 && R >= 2 * rhs) break;  // no tile narrower than half of R
-    int added = 0;  // the two widest windows that fit
+    // the period-aligned window, then the two widest powers of two that fit
+    const int aligned = lane_window(p, R, tsize);
+    for (int block : {1024, 512, 256})
+      add(1, 1, R, block, aligned, 0, SPTRSV_ENGINE_LANE);
+    int added = 0;
     for (int window : {8192, 4096, 2048, 1024}) {
       if (lane_lds_bytes(window, R, tsize) > p->lds_max) continue;
-      for (int block : {1024, 512, 256})
-        add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
+      if (window != aligned)
+        for (int block : {1024, 512, 256})
+          add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
       if (++added == 2) break;
     }
   }

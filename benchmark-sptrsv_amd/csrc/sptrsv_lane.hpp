@@ -881,6 +881,24 @@ __global__ void k_batch_level(const int *__restrict__ colPtr,
   }
 }
 
+// Reach of sampled columns: out[s] = the largest |row - column| among the
+// entries of column s * stride (0 for a diagonal-only column).  The most
+// common reach is the factor's period (a grid plane or row), to which the
+// default windows are aligned.
+__global__ void k_sample_reach(const int *__restrict__ colPtr,
+                               const int *__restrict__ rowIdx, int n,
+                               int stride, int nsamp, int *out) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nsamp;
+       s += gridDim.x * blockDim.x) {
+    const int c = s * stride;
+    int reach = 0;
+    if (c < n)
+      for (int j = colPtr[c]; j < colPtr[c + 1]; j++)
+        reach = max(reach, abs(rowIdx[j] - c));
+    out[s] = reach;
+  }
+}
+
 __global__ void k_need_remote(const int2 *__restrict__ need, int n, int *out) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += gridDim.x * blockDim.x)

@@ -217,3 +217,16 @@ def test_batch_claim_order_largest_first(backward):
     for cfg in (None, dict(engine=LANE, block=512, window=2048)):
         x, _ = plan_solve(f, b, subst=backward, cfg=cfg)
         assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
+
+
+def test_default_windows_follow_the_grid_period():
+    # the default lane window holds whole planes (or rows) of a grid: a
+    # 60x50x30 grid's plane is 3000 columns, so windows of 6000; a plane
+    # wider than 8192 columns gets a window of exactly one plane
+    for dims, window in (((60, 50, 30), 6000), ((100, 100, 12), 10000), ((128, 128, 8), 8192)):
+        f = mx.laplacian3d_lower(*dims)
+        plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
+        assert plan.get_config(1)["window"] == window, dims
+        b = f.spmv(mx.x_uniform(f.n, seed=3))
+        x, _ = plan_solve(f, b, plan=plan)
+        assert_close(x, oracle_x(f, b), 1e-12)
