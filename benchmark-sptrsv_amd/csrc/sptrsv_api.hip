@@ -422,25 +422,25 @@ struct sptrsv_plan_s {
 
 namespace {
 
-// Lane-engine window for R right-hand sides: aligned to the factor's period
-// (whole grid rows or planes per window, so that no window holds the tail of
-// one plane and the head of the next: such a window waits on both at once,
-// and a 100^3 grid took 5.8 ms with 8192-column windows against 0.32 ms with
-// plane-aligned ones), otherwise the widest power of two up to 8192 that
-// fits in LDS.  Batches of systems keep the power of two (their windows
-// follow the systems).
+// Lane-engine window (the largest window size) for R right-hand sides: the
+// widest power of two up to 8192 that fits in LDS, or, for a period (grid
+// plane) wider than that, one period when it fits in LDS.  The windows
+// themselves follow the period (get_windows): whole periods, or each period
+// split evenly, so that no window holds the tail of one plane and the head
+// of the next -- such a window waits on both at once (a 100^3 grid took
+// 5.8 ms with 8192-column windows against 0.32 ms with plane-aligned ones).
+// Batches of systems keep the power of two (their windows follow the
+// systems).
 int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
   int pow2 = 8192;
   while (pow2 > 256 && lane_lds_bytes(pow2, R, tsize) > p->lds_max) pow2 /= 2;
   const int D = p->period;
-  if (D <= 1 || p->nsys > 1) return pow2;
+  if (D <= pow2 || p->nsys > 1) return pow2;  // whole periods of <= 8192
   int wmax = 16384;  // the widest window that fits, a multiple of 64
   while (wmax > 256 && lane_lds_bytes(wmax, R, tsize) > p->lds_max) wmax -= 64;
-  if (D <= pow2) return D * (pow2 / D);
-  if (D <= wmax) return D;
-  for (int k = 2; k <= 64; k++)
-    if (D % k == 0 && D / k <= wmax) return D / k;
-  return pow2;
+  if (D <= wmax) return D;  // one period per window
+  const int k = (D + wmax - 1) / wmax;  // else each period split evenly
+  return (D + k - 1) / k;
 }
 
 sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
@@ -620,14 +620,39 @@ int get_windows(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet **out,
         st = SPTRSV_ERR_HIP;
       ws->has_order = st == SPTRSV_OK;
     }
+  } else if ((c.partition == SPTRSV_COL_PARTITION || n == 0) &&
+             p->period > c.window && p->period % c.window != 0) {
+    // every period (grid plane) split into nearly equal windows, so that no
+    // window holds the end of one plane and the start of the next
+    const int D = p->period;
+    const int k = (D + c.window - 1) / c.window;
+    std::vector<int> wp(1, 0);
+    for (long long q0 = 0; q0 < n; q0 += D)
+      for (int i = 1; i <= k; i++) {
+        const long long b = std::min<long long>(n, q0 + (long long)D * i / k);
+        if (b > wp.back()) wp.push_back((int)b);
+      }
+    ws->nwin = (int)wp.size() - 1;
+    ws->maxw = 0;
+    for (int q = 0; q < ws->nwin; q++)
+      ws->maxw = std::max(ws->maxw, wp[q + 1] - wp[q]);
+    st = ws->winPtr.ensure(sizeof(int) * wp.size(), false);
+    if (st == SPTRSV_OK &&
+        hipMemcpy(ws->winPtr.p, wp.data(), sizeof(int) * wp.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      st = SPTRSV_ERR_HIP;
   } else if (c.partition == SPTRSV_COL_PARTITION || n == 0) {
-    ws->uniform_w = c.window;
-    ws->nwin = n == 0 ? 0 : (n + c.window - 1) / c.window;
-    ws->maxw = std::min(c.window, std::max(n, 1));
+    // uniform windows; with a period no wider than the window, whole periods
+    int w = c.window;
+    if (p->period > 1 && p->period <= c.window && p->nsys == 1)
+      w = p->period * (c.window / p->period);
+    ws->uniform_w = w;
+    ws->nwin = n == 0 ? 0 : (n + w - 1) / w;
+    ws->maxw = std::min(w, std::max(n, 1));
     st = ws->winPtr.ensure(sizeof(int) * ((size_t)ws->nwin + 1), false);
     if (st == SPTRSV_OK) {
       k_uniform_windows<<<grid_for(ws->nwin + 1, 256), 256, 0, s>>>(
-          ws->winPtr.as<int>(), ws->nwin, c.window, n);
+          ws->winPtr.as<int>(), ws->nwin, w, n);
     }
   } else {
     // nnz-balanced windows over columns in solve order

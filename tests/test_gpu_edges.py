@@ -220,10 +220,12 @@ def test_batch_claim_order_largest_first(backward):
 
 
 def test_default_windows_follow_the_grid_period():
-    # the default lane window holds whole planes (or rows) of a grid: a
-    # 60x50x30 grid's plane is 3000 columns, so windows of 6000; a plane
-    # wider than 8192 columns gets a window of exactly one plane
-    for dims, window in (((60, 50, 30), 6000), ((100, 100, 12), 10000), ((128, 128, 8), 8192)):
+    # the default lane windows hold whole planes (or rows) of a grid: planes
+    # up to 8192 columns are packed whole into 8192-column windows; a wider
+    # plane gets a window of exactly one plane when it fits in LDS, else it
+    # is split evenly (16384 -> 2 x 8192, 127^2 = 16129 -> 8065 + 8064)
+    for dims, window in (((60, 50, 30), 8192), ((100, 100, 12), 10000),
+                         ((128, 128, 8), 8192), ((127, 127, 9), 8065)):
         f = mx.laplacian3d_lower(*dims)
         plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
         assert plan.get_config(1)["window"] == window, dims
