@@ -446,9 +446,12 @@ int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
 sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   sptrsv_config_t c;
   memset(&c, 0, sizeof(c));
-  if (p->far_frac >= 0.5) {
-    // most entries cross any LDS window: pull over the CSR, a wavefront per
-    // row (power-law hubs have hundreds of entries)
+  if (p->far_frac >= 0.5 && p->period == 0) {
+    // most entries cross any LDS window and follow no grid period: pull
+    // over the CSR, a wavefront per row (power-law hubs have hundreds of
+    // entries).  A grid whose planes are wider than the far distance (a
+    // 27-point stencil on 100 x 100 planes) keeps the lane engine: its
+    // windows follow the planes and its remote inputs arrive in order
     c.engine = SPTRSV_ENGINE_ROW;
     c.lanes_per_column = 64;
     c.rhs_lanes = 1;
@@ -739,6 +742,7 @@ T slot_empty_host() {
 // table (ws->bstart window-local, ws->boff) and, for the analysis, each
 // batch's first position (bfirst) and each position's batch (bid).
 int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
+                  std::vector<int> *boff_h, std::vector<int> *bord_h,
                   hipStream_t s) {
   const int n = p->n;
   std::vector<int> wp(ws->nwin + 1, 0);
@@ -774,7 +778,22 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
     long links = 0;
     for (int q = wlo + 1; q < whi; q++) links += link[q] != 0;
     const bool chained = 2 * links >= (long)(whi - wlo - 1) && whi - wlo > 1;
+    // run[q - wlo]: length of the chain run (columns joined by links) that
+    // starts at q
+    std::vector<int> run(whi - wlo, 1);
+    for (int q = whi - 2; q >= wlo; q--)
+      if (link[q + 1]) run[q - wlo] = run[q + 1 - wlo] + 1;
+    // a run longer than a wavefront is cut into equal pieces (a 70-column
+    // grid row into 35 + 35, not 64 + 6)
+    auto piece_of = [&](int q) {
+      if (!chained) return kWave;
+      const int len = run[q - wlo];
+      if (len <= kWave) return kWave;
+      const int k = (len + kWave - 1) / kWave;
+      return (len + k - 1) / k;
+    };
     int start = wlo;
+    int piece = piece_of(wlo);
     bool pure = true;  // every link inside the current batch is pure
     local.push_back(0);
     first.push_back(wlo);
@@ -783,9 +802,15 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
       // a run of pure links grows past a wavefront into a wide batch of up
       // to kWave * kWideC columns
       const bool wide_ok = pure && link[q] == 2 && len < kWave * kWideC;
-      if (q > wlo && ((len >= kWave && !wide_ok) ||
-                      (chained && !link[q] && len >= kMinChainBatch))) {
+      // at a chain break (a grid row's start) short runs merge into one
+      // batch, but a batch never reaches into a run of kMinChainBatch or
+      // more columns: batches stay aligned with the rows after a window
+      // that starts mid-row
+      const bool brk = chained && !link[q] &&
+                       (len >= kMinChainBatch || run[q - wlo] >= kMinChainBatch);
+      if (q > wlo && ((len >= piece && !wide_ok) || brk)) {
         start = q;
+        if (!link[q]) piece = piece_of(q);
         pure = true;
         local.push_back(q - wlo);
         first.push_back(q);
@@ -819,24 +844,40 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(bid->p, owner.data(), sizeof(int) * owner.size(),
                          hipMemcpyHostToDevice, s));
-  // processing order: each window's batches by (level in the window's batch
-  // DAG, index).  Every batch's in-window predecessors have lower levels, so
-  // the wavefront owning the lowest unfinished batch of that order has
-  // finished its earlier ones and that batch's inputs are complete: the
-  // round-robin deal of this order cannot deadlock, and wavefronts take
-  // batches in the order they become ready (anti-diagonals of a grid)
+  // processing order: each window's batches by (level in the batch DAG of
+  // the whole factor, index).  Every batch's in-window predecessors have
+  // lower levels, so the wavefront owning the lowest unfinished batch of
+  // that order has finished its earlier ones and that batch's inputs are
+  // complete: the round-robin deal of this order cannot deadlock.  Levels
+  // of the whole factor, not of the window alone, put first the batches
+  // whose inputs arrive first: a window that holds the tail of one grid
+  // plane and the head of the next gets the next plane's rows (inputs from
+  // the previous window's head) ahead of this plane's (inputs from its
+  // tail).  Batches are contiguous in solve order, so one pass over the
+  // columns in that order sees every batch's level final before it pushes.
   std::vector<int> lev(nb, 0);
-  if (n > 0 && !local.empty() && ws->nwin > 0) {
-    DevBuf dlev;
-    ST_TRY(dlev.ensure(sizeof(int) * local.size(), true));
-    k_batch_level<<<ws->nwin, kWave, 0, s>>>(
-        p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
-        bfirst->as<int>(), ws->boff.as<int>(), ws->winPtr.as<int>(),
-        bid->as<int>(), dlev.as<int>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(lev.data(), dlev.p, sizeof(int) * local.size(),
+  if (n > 0 && !local.empty()) {
+    std::vector<int> cp((size_t)n + 1), ri((size_t)std::max(p->nnz, 1));
+    HIP_TRY(hipMemcpyAsync(cp.data(), p->colPtr, sizeof(int) * cp.size(),
                            hipMemcpyDeviceToHost, s));
+    if (p->nnz > 0)
+      HIP_TRY(hipMemcpyAsync(ri.data(), p->rowIdx, sizeof(int) * (size_t)p->nnz,
+                             hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const bool bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
+    for (int q = 0; q < n; q++) {
+      const int c = bwd ? n - 1 - q : q;
+      const int b = owner[q];
+      const int lb = lev[b] + 1;
+      const int f = cp[c], l = cp[c + 1];
+      const int lo = bwd ? f : f + 1, hi = bwd ? l - 1 : l;
+      for (int j = lo; j < hi; j++) {
+        const int r = ri[j];
+        if (r < 0 || r >= n) continue;
+        const int br = owner[bwd ? n - 1 - r : r];
+        if (br != b && lev[br] < lb) lev[br] = lb;
+      }
+    }
   }
   std::vector<int> order(nb, 0);
   for (int w = 0; w < ws->nwin; w++) {
@@ -852,6 +893,8 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   HIP_TRY(hipMemcpyAsync(ws->bord.p, order.data(), sizeof(int) * nb,
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
+  *boff_h = std::move(off);
+  *bord_h = std::move(order);
   return SPTRSV_OK;
 }
 
@@ -892,7 +935,8 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
   // k_lane_targets, whose etgt values it overwrites for the selected entries)
   ST_TRY(ws->chain.ensure(sizeof(int2) * (size_t)std::max(n, 1), false));
   DevBuf bfirst, bid;
-  ST_TRY(build_batches(p, ws, &bfirst, &bid, s));
+  std::vector<int> boff_h, bord_h;
+  ST_TRY(build_batches(p, ws, &bfirst, &bid, &boff_h, &bord_h, s));
   if (n > 0) {
     DevBuf key;
     ST_TRY(key.ensure(sizeof(unsigned long long) * (size_t)n, true));
@@ -914,11 +958,13 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     key.release();
-    bfirst.release();
-    bid.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
   }
-  // receive lists: the columns with remote inputs, grouped by window
+  // receive lists: the columns with remote inputs, grouped by window and
+  // within a window in the batches' processing order, so receivers (which
+  // finish their entries in list order) take a row's arrivals when its
+  // batch is about to be solved; by row order, rows of the next grid plane
+  // at a window's end would wait behind every earlier row of the window
   {
     DevBuf pos;
     ST_TRY(pos.ensure(sizeof(int) * ((size_t)n + 1), false));
@@ -940,9 +986,39 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
       st = ws->rlist.ensure(sizeof(int4) * (size_t)std::max(nrecv_cols, 1), false);
     if (st == SPTRSV_OK)
       st = ws->roff.ensure(sizeof(int) * ((size_t)ws->nwin + 1), false);
+    // shift[b]: where batch b's entries move (processing order) minus where
+    // they start in solve order
+    const int nbt = (int)bord_h.size();
+    DevBuf bpos, shift;
+    if (st == SPTRSV_OK) st = bpos.ensure(sizeof(int) * ((size_t)nbt + 1), false);
+    if (st == SPTRSV_OK) st = shift.ensure(sizeof(int) * (size_t)nbt, false);
+    if (st == SPTRSV_OK && n > 0 && boff_h.back() > 0) {
+      k_batch_entry_pos<<<grid_for(nbt + 1, 256), 256, 0, s>>>(
+          bfirst.as<int>(), pos.as<int>(), boff_h.back(), n, bpos.as<int>());
+      std::vector<int> g(boff_h.back() + 1), sh(nbt, 0);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(g.data(), bpos.p, sizeof(int) * g.size(),
+                         hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        st = SPTRSV_ERR_HIP;
+      for (int w = 0; st == SPTRSV_OK && w < ws->nwin; w++) {
+        int cur = g[boff_h[w]];
+        for (int i = boff_h[w]; i < boff_h[w + 1]; i++) {
+          const int b = boff_h[w] + bord_h[i];
+          sh[b] = cur - g[b];
+          cur += g[b + 1] - g[b];
+        }
+      }
+      if (st == SPTRSV_OK &&
+          (hipMemcpyAsync(shift.p, sh.data(), sizeof(int) * sh.size(),
+                          hipMemcpyHostToDevice, s) != hipSuccess ||
+           hipStreamSynchronize(s) != hipSuccess))
+        st = SPTRSV_ERR_HIP;
+    }
     if (st == SPTRSV_OK) {
       k_receive_list<<<grid_for(std::max(n, ws->nwin + 1), 256), 256, 0, s>>>(
           ws->need.as<int2>(), ws->rptr.as<int>(), pos.as<int>(),
+          bid.as<int>(), shift.as<int>(),
           ws->winPtr.as<int>(), ws->nwin, n,
           p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
           ws->uniform_w,
@@ -951,6 +1027,10 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
         st = SPTRSV_ERR_HIP;
     }
     pos.release();
+    bpos.release();
+    shift.release();
+    bfirst.release();
+    bid.release();
     if (st != SPTRSV_OK) return st;
   }
   ws->nslots = nslots;

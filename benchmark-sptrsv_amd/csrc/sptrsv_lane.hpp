@@ -528,45 +528,56 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 #pragma unroll
               for (int q = 0; q < R; q++)
                 add_wg(&s_val[ladr[e] * R + q], -(av[q] * lval[e]));
-            // entries past the registers, kPF at a time (loads issued
-            // together, one wait per group)
-            for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
-              int tt[kPF];
-              T tv[kPF];
-#pragma unroll
-              for (int i = 0; i < kPF; i++) {
-                tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
-                tv[i] = j0 + i < hi ? a.val[j0 + i] : T(0);
-              }
-#pragma unroll
-              for (int i = 0; i < kPF; i++) {
-                const int t = tt[i];
-                if (t >= 0) {
-#pragma unroll
-                  for (int q = 0; q < R; q++)
-                    if (q < rv)
-                      xchg_agent(&slots[(size_t)t * R + q], av[q] * tv[i]);
-                } else if (t != kRegTarget) {
-#pragma unroll
-                  for (int q = 0; q < R; q++)
-                    if (q < rv)
-                      add_wg(&s_val[(-t - 1) * R + q], -(av[q] * tv[i]));
-                }
-              }
-            }
           }
+          // the register entries' rows (the next grid row of a stencil) are
+          // released before the entries past the registers are loaded
           fence_rel_wg();
           if (solved) {
 #pragma unroll
             for (int e = 0; e < kPF; e++) add_wg(&s_cnt[ladr[e]], linc[e]);
-            for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
-              int tt[kPF];
+          }
+          const bool longc = solved && hi - lo > kPF;
+          if (__ballot(longc) != 0) {
+            if (longc) {
+              // entries past the registers, kPF at a time (loads issued
+              // together, one wait per group)
+              for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
+                int tt[kPF];
+                T tv[kPF];
 #pragma unroll
-              for (int i = 0; i < kPF; i++)
-                tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
+                for (int i = 0; i < kPF; i++) {
+                  tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
+                  tv[i] = j0 + i < hi ? a.val[j0 + i] : T(0);
+                }
 #pragma unroll
-              for (int i = 0; i < kPF; i++)
-                if (tt[i] < 0 && tt[i] != kRegTarget) add_wg(&s_cnt[-tt[i] - 1], 1);
+                for (int i = 0; i < kPF; i++) {
+                  const int t = tt[i];
+                  if (t >= 0) {
+#pragma unroll
+                    for (int q = 0; q < R; q++)
+                      if (q < rv)
+                        xchg_agent(&slots[(size_t)t * R + q], av[q] * tv[i]);
+                  } else if (t != kRegTarget) {
+#pragma unroll
+                    for (int q = 0; q < R; q++)
+                      if (q < rv)
+                        add_wg(&s_val[(-t - 1) * R + q], -(av[q] * tv[i]));
+                  }
+                }
+              }
+            }
+            fence_rel_wg();
+            if (longc) {
+              for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
+                int tt[kPF];
+#pragma unroll
+                for (int i = 0; i < kPF; i++)
+                  tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
+#pragma unroll
+                for (int i = 0; i < kPF; i++)
+                  if (tt[i] < 0 && tt[i] != kRegTarget)
+                    add_wg(&s_cnt[-tt[i] - 1], 1);
+              }
             }
           }
         }
@@ -844,43 +855,6 @@ __global__ void k_chain_mark(const unsigned long long *__restrict__ key, int n,
   }
 }
 
-// Batch levels of one window per workgroup of one wavefront: the batches
-// are visited in index order, which is topological (every entry goes from a
-// column to a later row), so a batch's level is final when it is visited;
-// its columns then push level + 1 to the batches of their rows in the same
-// window (in-batch links weigh nothing; cross-window inputs are not
-// counted).
-__global__ void k_batch_level(const int *__restrict__ colPtr,
-                              const int *__restrict__ rowIdx, int n,
-                              int backward, const int *__restrict__ bfirst,
-                              const int *__restrict__ boff,
-                              const int *__restrict__ winPtr,
-                              const int *__restrict__ bid, int *lev) {
-  const int w = blockIdx.x;
-  const int b0 = boff[w], b1 = boff[w + 1];
-  const int wend = winPtr[w + 1];
-  for (int bi = b0; bi < b1; bi++) {
-    // the pushes of earlier batches are complete and visible (atomics at the
-    // coherence point, read with an agent-scope load)
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    const int lb = ld_agent(&lev[bi]) + 1;
-    const int ps = bfirst[bi], pe = bi + 1 < b1 ? bfirst[bi + 1] : wend;
-    for (int q = ps + (int)threadIdx.x; q < pe; q += blockDim.x) {
-      const int c = backward ? n - 1 - q : q;
-      const int f = colPtr[c], l = colPtr[c + 1];
-      const int lo = backward ? f : f + 1, hi = backward ? l - 1 : l;
-      for (int j = lo; j < hi; j++) {
-        const int r = rowIdx[j];
-        const int pr = backward ? n - 1 - r : r;
-        if (pr < wend) {
-          const int br = bid[pr];
-          if (br != bi) atomicMax(&lev[br], lb);
-        }
-      }
-    }
-  }
-}
-
 // Reach of sampled columns: out[s] = the largest |row - column| among the
 // entries of column s * stride (0 for a diagonal-only column).  The most
 // common reach is the factor's period (a grid plane or row), to which the
@@ -918,13 +892,25 @@ __global__ void k_remote_flags(const int2 *__restrict__ need, int n,
     count[p] = (need[backward ? n - 1 - p : p].y + kRecvChunk - 1) / kRecvChunk;
 }
 
+// bpos[b] = pos[first column of batch b] for b < nb; bpos[nb] = pos[n].
+__global__ void k_batch_entry_pos(const int *__restrict__ bfirst,
+                                  const int *__restrict__ pos, int nb, int n,
+                                  int *bpos) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= nb;
+       b += gridDim.x * blockDim.x)
+    bpos[b] = pos[b < nb ? bfirst[b] : n];
+}
+
 // Receive lists: pos = exclusive scan of the entry counts (solve order), so
-// each window's entries are contiguous and ascending; roff[w] =
-// pos[winPtr[w]].  Entry i of a column covers its slots
-// [rptr + i * kRecvChunk, ...), at most kRecvChunk of them.
+// each window's entries are contiguous; roff[w] = pos[winPtr[w]].  Within a
+// window a batch's entries (its columns in solve order) move by shift[batch]
+// to the batch's place in the processing order.  Entry i of a column covers
+// its slots [rptr + i * kRecvChunk, ...), at most kRecvChunk of them.
 __global__ void k_receive_list(const int2 *__restrict__ need,
                                const int *__restrict__ rptr,
                                const int *__restrict__ pos,
+                               const int *__restrict__ bid,
+                               const int *__restrict__ shift,
                                const int *__restrict__ winPtr, int nwin, int n,
                                int backward, int uniform_w, int4 *rlist,
                                int *roff) {
@@ -935,8 +921,9 @@ __global__ void k_receive_list(const int2 *__restrict__ need,
     if (m > 0) {
       const int w = uniform_w > 0 ? p / uniform_w
                                   : upper_bound_minus1(winPtr, nwin + 1, p);
+      const int e0 = pos[p] + shift[bid[p]];
       for (int i = 0, s = 0; s < m; i++, s += kRecvChunk)
-        rlist[pos[p] + i] =
+        rlist[e0 + i] =
             make_int4(p - winPtr[w], rptr[c] + s, min(kRecvChunk, m - s), 0);
     }
   }

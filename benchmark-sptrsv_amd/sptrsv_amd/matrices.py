@@ -96,6 +96,33 @@ def laplacian3d_lower(nx, ny, nz, diag=6.0, off=-1.0):
     return Factor(p, ri, vv, name=f"lap3d_{nx}x{ny}x{nz}")
 
 
+def stencil27_lower(nx, ny, nz, diag=26.0, off=-1.0):
+    """Lower triangle incl. diagonal of the 3D 27-point stencil (every grid
+    neighbour within one step along each axis): 13 entries below the
+    diagonal in an interior column, reaching one plane, one row and one
+    column ahead."""
+    n = nx * ny * nz
+    i = np.arange(n, dtype=np.int64)
+    x = i % nx
+    y = (i // nx) % ny
+    z = i // (nx * ny)
+    cols, rows, vals = [i], [i], [np.full(n, diag)]
+    for dz in (0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                d = dx + nx * dy + nx * ny * dz
+                if d <= 0:
+                    continue
+                ok = ((x + dx >= 0) & (x + dx < nx) & (y + dy >= 0) & (y + dy < ny)
+                      & (z + dz < nz))
+                cols.append(i[ok])
+                rows.append(i[ok] + d)
+                vals.append(np.full(int(ok.sum()), off))
+    p, ri, vv = _from_columns(n, np.concatenate(cols), np.concatenate(rows),
+                              np.concatenate(vals))
+    return Factor(p, ri, vv, name=f"stencil27_{nx}x{ny}x{nz}")
+
+
 def bidiagonal(n, diag=2.0, sub=-1.0):
     i = np.arange(n, dtype=np.int64)
     c = np.concatenate([i, i[:-1]])

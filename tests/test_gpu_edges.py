@@ -232,3 +232,32 @@ def test_default_windows_follow_the_grid_period():
         b = f.spmv(mx.x_uniform(f.n, seed=3))
         x, _ = plan_solve(f, b, plan=plan)
         assert_close(x, oracle_x(f, b), 1e-12)
+
+
+@pytest.mark.parametrize("backward", [0, 1])
+@pytest.mark.parametrize("dims,cfg", [
+    ((24, 20, 30), None),
+    ((24, 20, 30), dict(engine=LANE, block=1024, window=2048)),
+    ((70, 9, 20), dict(engine=LANE, block=512, window=4096)),
+    ((64, 6, 80), dict(engine=LANE, block=1024, window=2048)),
+])
+def test_stencil27_windows_starting_mid_row(dims, cfg, backward):
+    # the column reach of a 27-point factor is a plane + a row + 1, so
+    # period-aligned windows start at every column of a grid row in turn;
+    # batches must realign with the rows after a short first fragment, and
+    # rows wider than a wavefront split into equal pieces (70 -> 35 + 35)
+    f = mx.stencil27_lower(*dims)
+    if backward:
+        f = mx.upper_from_lower(f)
+    b = f.spmv(mx.x_uniform(f.n, seed=5))
+    x, _ = plan_solve(f, b, subst=backward, cfg=cfg)
+    assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
+
+
+@pytest.mark.parametrize("rhs", [3, 8])
+def test_stencil27_multi_rhs(rhs):
+    f = mx.stencil27_lower(30, 20, 16)
+    X = mx.x_normal(f.n, rhs, seed=6)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, rhs=rhs)
+    assert_close(x, oracle_x(f, B, rhs=rhs), 1e-12)

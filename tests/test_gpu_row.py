@@ -154,11 +154,14 @@ def test_row_engine_config_checks():
 
 def test_default_engine_follows_entry_distance():
     # most entries of this power-law factor lie >= 8192 columns below the
-    # diagonal (far fraction 0.74): the default is the row engine; stencils
-    # (fraction 0) and chains keep the lane engine
+    # diagonal (far fraction 0.74) with no common reach: the default is the
+    # row engine; stencils (fraction 0), chains, and a 27-point grid whose
+    # 10000-column planes put 9 of 13 entries beyond 8192 (but which has a
+    # period, the plane + a row + 1) keep the lane engine
     for f, engine in ((mx.power_law_lower(150_000, seed=2), ROW),
                       (mx.laplacian3d_lower(40, 40, 20), S.ENGINE_LANE),
-                      (mx.bidiagonal(20_000), S.ENGINE_LANE)):
+                      (mx.bidiagonal(20_000), S.ENGINE_LANE),
+                      (mx.stencil27_lower(100, 100, 5), S.ENGINE_LANE)):
         plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
         assert plan.get_config(1)["engine"] == engine, f.name
         b = f.spmv(mx.x_uniform(f.n, seed=4))

@@ -162,6 +162,18 @@ def test_spmv_matches_scipy():
     np.testing.assert_allclose(f.spmv(x), L @ x, rtol=1e-13, atol=1e-13)
 
 
+def test_stencil27_generator_matches_brute_force():
+    nx, ny, nz = 5, 4, 3
+    f = mx.stencil27_lower(nx, ny, nz)
+    n = nx * ny * nz
+    g = np.stack([np.arange(n) % nx, (np.arange(n) // nx) % ny, np.arange(n) // (nx * ny)], 1)
+    near = np.abs(g[:, None, :] - g[None, :, :]).max(axis=2) <= 1
+    M = np.where(near, -1.0, 0.0)
+    np.fill_diagonal(M, 26.0)
+    A = sp.csc_matrix((f.val, f.rowIdx, f.colPtr), shape=(n, n)).toarray()
+    assert np.array_equal(A, np.tril(M))
+
+
 def test_rel_err_formula():
     x = np.array([1.0, 2.0, -3.0])
     xr = np.array([1.0, 2.5, -2.0])
@@ -175,6 +187,9 @@ def test_rel_err_formula():
     (mx.laplacian2d_lower(17, 5), 21),
     (mx.laplacian3d_lower(9, 7, 5), 19),
     (mx.bidiagonal(1000), 1000),
+    # 27-point: level(x, y, z) = x + 2y + 4z (every unit step back lowers it by 1)
+    (mx.stencil27_lower(5, 4, 3), 4 + 6 + 8 + 1),
+    (mx.stencil27_lower(9, 7, 6), 8 + 12 + 20 + 1),
 ])
 def test_findlevel_closed_forms(f, expect):
     nl, pmin, pavg, pmax = orc.findlevel(f.colPtr, f.rowIdx)
