@@ -86,6 +86,8 @@ struct LaneArgs {
 };
 
 constexpr int kPF = 4;  // off-diagonal entries per column held in registers
+// entries past the registers loaded per group when a column publishes
+constexpr int kLong = 4;
 
 // Cross-lane move with DPP (no LDS): lanes whose source is outside the
 // row or whose row is masked off get `old`.
@@ -536,48 +538,38 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 #pragma unroll
             for (int e = 0; e < kPF; e++) add_wg(&s_cnt[ladr[e]], linc[e]);
           }
-          const bool longc = solved && hi - lo > kPF;
-          if (__ballot(longc) != 0) {
-            if (longc) {
-              // entries past the registers, kPF at a time (loads issued
-              // together, one wait per group)
-              for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
-                int tt[kPF];
-                T tv[kPF];
+          // entries past the registers, kLong at a time: loads issued
+          // together, one wait per group; each group's LDS values are
+          // released (an LDS wait) before its counts
+          if (solved && hi - lo > kPF) {
+            for (int j0 = lo + kPF; j0 < hi; j0 += kLong) {
+              int tt[kLong];
+              T tv[kLong];
 #pragma unroll
-                for (int i = 0; i < kPF; i++) {
-                  tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
-                  tv[i] = j0 + i < hi ? a.val[j0 + i] : T(0);
-                }
+              for (int i = 0; i < kLong; i++) {
+                tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
+                tv[i] = j0 + i < hi ? a.val[j0 + i] : T(0);
+              }
 #pragma unroll
-                for (int i = 0; i < kPF; i++) {
-                  const int t = tt[i];
-                  if (t >= 0) {
+              for (int i = 0; i < kLong; i++) {
+                const int t = tt[i];
+                if (t >= 0) {
 #pragma unroll
-                    for (int q = 0; q < R; q++)
-                      if (q < rv)
-                        xchg_agent(&slots[(size_t)t * R + q], av[q] * tv[i]);
-                  } else if (t != kRegTarget) {
+                  for (int q = 0; q < R; q++)
+                    if (q < rv)
+                      xchg_agent(&slots[(size_t)t * R + q], av[q] * tv[i]);
+                } else if (t != kRegTarget) {
 #pragma unroll
-                    for (int q = 0; q < R; q++)
-                      if (q < rv)
-                        add_wg(&s_val[(-t - 1) * R + q], -(av[q] * tv[i]));
-                  }
+                  for (int q = 0; q < R; q++)
+                    if (q < rv)
+                      add_wg(&s_val[(-t - 1) * R + q], -(av[q] * tv[i]));
                 }
               }
-            }
-            fence_rel_wg();
-            if (longc) {
-              for (int j0 = lo + kPF; j0 < hi; j0 += kPF) {
-                int tt[kPF];
+              fence_rel_wg();
 #pragma unroll
-                for (int i = 0; i < kPF; i++)
-                  tt[i] = j0 + i < hi ? a.etgt[j0 + i] : kRegTarget;
-#pragma unroll
-                for (int i = 0; i < kPF; i++)
-                  if (tt[i] < 0 && tt[i] != kRegTarget)
-                    add_wg(&s_cnt[-tt[i] - 1], 1);
-              }
+              for (int i = 0; i < kLong; i++)
+                if (tt[i] < 0 && tt[i] != kRegTarget)
+                  add_wg(&s_cnt[-tt[i] - 1], 1);
             }
           }
         }
