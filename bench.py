@@ -54,6 +54,10 @@ def make_workload(name):
     if name == "batch":
         f = mx.laplacian2d_batch(512, seed=0)
         return f, mx.x_uniform(f.n, seed=0), None
+    if name == "stencil27":
+        # level(x, y, z) = x + 2y + 4z
+        f = mx.stencil27_lower(100, 100, 60)
+        return f, mx.x_uniform(f.n, seed=0), 99 + 2 * 99 + 4 * 59 + 1
     if name == "lap3d_share":
         # one GPU's share of the 3D Laplacian under an 8-way contiguous column
         # split: the diagonal block of 16 planes.  Its right-hand side is
@@ -70,6 +74,7 @@ WORKLOAD_DESC = {
     "powerlaw": "random power-law lower triangular n=1,000,000, Zipf(2) cap 1000, window 50,000",
     "bidiag": "bidiagonal chain n=1,000,000, diag 2, sub -1",
     "batch": "512 uneven 2D 5-point Laplacian lower factors (sides log-U[32,256]) in one launch",
+    "stencil27": "lower triangle of 3D 27-point stencil 100x100x60 (n=600,000), diag 26, off -1",
     "lap3d_share": "one GPU's share of the 128^3 3D Laplacian factor under an 8-way "
                    "contiguous column split (diagonal block of 16 planes, n=262,144; "
                    "cross-block contributions pre-accumulated into b)",
