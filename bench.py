@@ -176,7 +176,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path "
-              "(a single triangular system is one GPU's work; see DESIGN.md)",
+              "(a single triangular system is one GPU's work; independent "
+              "systems or right-hand-side tiles would shard with no exchange)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -783,17 +783,7 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
     std::vector<int> run(whi - wlo, 1);
     for (int q = whi - 2; q >= wlo; q--)
       if (link[q + 1]) run[q - wlo] = run[q + 1 - wlo] + 1;
-    // a run longer than a wavefront is cut into equal pieces (a 70-column
-    // grid row into 35 + 35, not 64 + 6)
-    auto piece_of = [&](int q) {
-      if (!chained) return kWave;
-      const int len = run[q - wlo];
-      if (len <= kWave) return kWave;
-      const int k = (len + kWave - 1) / kWave;
-      return (len + k - 1) / k;
-    };
     int start = wlo;
-    int piece = piece_of(wlo);
     bool pure = true;  // every link inside the current batch is pure
     local.push_back(0);
     first.push_back(wlo);
@@ -808,9 +798,8 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
       // that starts mid-row
       const bool brk = chained && !link[q] &&
                        (len >= kMinChainBatch || run[q - wlo] >= kMinChainBatch);
-      if (q > wlo && ((len >= piece && !wide_ok) || brk)) {
+      if (q > wlo && ((len >= kWave && !wide_ok) || brk)) {
         start = q;
-        if (!link[q]) piece = piece_of(q);
         pure = true;
         local.push_back(q - wlo);
         first.push_back(q);

@@ -540,8 +540,11 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           }
           // entries past the registers, kLong at a time: loads issued
           // together, one wait per group; each group's LDS values are
-          // released (an LDS wait) before its counts
-          if (solved && hi - lo > kPF) {
+          // released (an LDS wait) before its counts.  A wavefront with no
+          // long column skips the section on a scalar branch (without it
+          // the 2D grids of 70-100 columns lose 2-3%)
+          const bool longc = solved && hi - lo > kPF;
+          if (__ballot(longc) != 0 && longc) {
             for (int j0 = lo + kPF; j0 < hi; j0 += kLong) {
               int tt[kLong];
               T tv[kLong];

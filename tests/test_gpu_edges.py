@@ -244,8 +244,8 @@ def test_default_windows_follow_the_grid_period():
 def test_stencil27_windows_starting_mid_row(dims, cfg, backward):
     # the column reach of a 27-point factor is a plane + a row + 1, so
     # period-aligned windows start at every column of a grid row in turn;
-    # batches must realign with the rows after a short first fragment, and
-    # rows wider than a wavefront split into equal pieces (70 -> 35 + 35)
+    # batches must realign with the rows after a short first fragment (also
+    # for rows wider than a wavefront: 70 = 64 + 6)
     f = mx.stencil27_lower(*dims)
     if backward:
         f = mx.upper_from_lower(f)
