@@ -113,7 +113,18 @@ def per_system_launches(f, b, repeats=3):
     return ms, [p[2] for p in plans], [(p[3], p[4]) for p in plans]
 
 
-def cpu_baseline(f, b, budget_s, substitution=0):
+def host_cpu():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(f, b, budget_s, substitution=0, rhs=1):
     """Single-threaded serial sync-free executor on the same matrix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import sptrsv_oracle as orc
@@ -125,14 +136,16 @@ def cpu_baseline(f, b, budget_s, substitution=0):
     times = []
     while time.perf_counter() - t0 < budget_s or reps < 2:
         t = time.perf_counter()
-        x_cpu = fn(colPtr, rowIdx, f.val, b, substitution=substitution, in_degree=indeg)
+        x_cpu = fn(colPtr, rowIdx, f.val, b, substitution=substitution, rhs=rhs,
+                   in_degree=indeg)
         times.append(time.perf_counter() - t)
         reps += 1
     med = float(np.median(times))
-    return {"value": round(1.0 / med, 3), "unit": "solves/s", "cores": 1, "kind": kind,
+    return {"value": round(rhs / med, 3), "unit": "solves/s", "cores": 1, "kind": kind,
             "sample": f"{reps} serial FP64 executor runs on the same matrix and b "
-                      f"(median {med * 1e3:.2f} ms each, analysis excluded), "
-                      f"{os.cpu_count()} host cores visible"}, x_cpu
+                      f"({rhs} right-hand side{'s' if rhs > 1 else ''}; median "
+                      f"{med * 1e3:.2f} ms each, analysis excluded) on 1 core of "
+                      f"{host_cpu()}, {os.cpu_count()} host cores visible"}, x_cpu
 
 
 def main():
@@ -155,6 +168,13 @@ def main():
     ap.add_argument("--form", default="plain", choices=["plain", "ir"])
     ap.add_argument("--ir-iters", type=int, default=2)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--config", default=None,
+                    help='executor configuration as JSON (e.g. \'{"engine": 2, '
+                         '"lanes_per_column": 64, "rhs_lanes": 64, "rhs_per_lane": 2}\'); '
+                         "skips the autotuner")
+    ap.add_argument("--graph", default=None, action=argparse.BooleanOptionalAction,
+                    help="time replays of a hipGraph of the solve (default: on for "
+                         "the batch workload, whose configuration names one graph launch)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -202,9 +222,8 @@ def main():
         if bwd:
             from sptrsv_amd import matrices as mx
             f = mx.upper_from_lower(f)
-    if levels is None:
-        # DAG depth of the factor (findlevel.h's definition), host analysis
-        levels = S.levels(f.colPtr, f.rowIdx)[0]
+    if args.graph is None:
+        args.graph = args.workload == "batch"
     rhs = args.rhs
     if rhs > 1:
         from sptrsv_amd import matrices as mx
@@ -214,21 +233,40 @@ def main():
         f"(generated in {time.perf_counter() - t:.1f} s)")
 
     flags = S.PLAN_VALIDATE | (S.PLAN_MIXED if args.form == "ir" else 0)
+    mode = 1 if args.form == "ir" else 0
     dP = S.DeviceArray.from_host(f.colPtr)
     dI = S.DeviceArray.from_host(f.rowIdx)
     dV = S.DeviceArray.from_host(f.val)
-    plan = S.Plan(dP, dI, dV, f.n, f.nnz, substitution=int(bwd), flags=flags,
-                  sys_ptr=f.sys_ptr)
+
+    def new_plan():
+        return S.Plan(dP, dI, dV, f.n, f.nnz, substitution=int(bwd), flags=flags,
+                      sys_ptr=f.sys_ptr)
+
     db = S.DeviceArray.from_host(b)
     dx = S.DeviceArray(b.shape, np.float64)
     stream = S.Stream()
     tuned = None
-    if not args.no_tune:
-        tuned, tms = plan.autotune(rhs=rhs, mode=1 if args.form == "ir" else 0)
+    if args.config:
+        tuned = json.loads(args.config)
+    elif not args.no_tune:
+        tplan = new_plan()
+        tuned, tms = tplan.autotune(rhs=rhs, mode=mode)
         log(f"autotuned config {tuned} ({tms:.3f} ms)")
+        tplan.destroy()
+    # the timed plan: created afresh, so its analysis times are those of the
+    # configuration that is timed (plan creation + per-configuration analysis)
+    plan = new_plan()
+    if tuned:
+        plan.set_config(rhs, **tuned)
+    plan.prepare(rhs, mode, stream=stream)
+    graph = None
+    if args.graph:
+        graph = plan.graph(db, dx, rhs=rhs, mode=mode, iters=args.ir_iters, stream=stream)
 
     def step():
-        if args.form == "ir":
+        if graph is not None:
+            graph.launch(stream)
+        elif args.form == "ir":
             plan.solve_ir(db, dx, rhs=rhs, iters=args.ir_iters, stream=stream)
         else:
             plan.solve(db, dx, rhs=rhs, stream=stream)
@@ -274,6 +312,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
+    if levels is None:
+        # DAG depth of the factor (findlevel.h's definition), device analysis
+        levels = plan.levels(stream)[0]
     pinfo = plan.info()
     out = {
         "metric": METRIC,
@@ -296,7 +337,10 @@ def main():
                            else f"FP32 solve + {args.ir_iters} FP64 refinement steps"),
             "n": f.n, "nnz": f.nnz, "nrhs": rhs, "dag_levels": levels,
             "executor": tuned or plan.get_config(rhs),
-            "analysis_ms": round(pinfo["analysis_ms"], 3),
+            "timed_as": "hipGraph replays" if graph is not None else "eager launches",
+            "analysis_ms": round(pinfo["analysis_ms"] + pinfo["prepare_ms"], 3),
+            "analysis_split_ms": {"plan_create": round(pinfo["analysis_ms"], 3),
+                                  "per_configuration": round(pinfo["prepare_ms"], 3)},
         },
         "roofline": {
             "bound": "hbm",
@@ -323,8 +367,8 @@ def main():
             "batched_speedup": round(ps_ms / ms_step, 2),
             "rel_err_vs_x_true_linf": worst,
             "note": "same systems, one plan and one launch each, default executor"}
-    if not args.no_cpu and rhs == 1:
-        cpu, x_cpu = cpu_baseline(f, b.reshape(-1), args.cpu_seconds, int(bwd))
+    if not args.no_cpu:
+        cpu, x_cpu = cpu_baseline(f, b.reshape(-1), args.cpu_seconds, int(bwd), rhs)
         out["cpu_baseline"] = cpu
         # the CPU run is also the checker: same matrix, same b
         d = np.abs(x.reshape(-1) - x_cpu)

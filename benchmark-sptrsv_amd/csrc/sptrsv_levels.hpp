@@ -1,0 +1,165 @@
+// sptrsv_levels.hpp -- level analysis of the dependency DAG on the device.
+//
+// The level of a row is the length of the longest dependency path that ends
+// at it: 0 for a row whose only entry is the diagonal, otherwise one more
+// than the largest level among the rows it reads.  That is the level set of
+// findlevel_csc (SpTRSV_cuda/findlevel.h:6-69: a Kahn sweep from the rows of
+// in-degree 1, each row entering the level after its last predecessor's), and
+// it is itself a sync-free triangular solve in the (max, +) semiring, so it
+// is computed the way the row engine solves: a pull over the CSR where each
+// level is written once with an exchange and is its own arrival flag (-1
+// while unset).
+//
+// The same kernel gives the levels of a coarser DAG whose nodes ("units") are
+// runs of consecutive solve positions -- the lane engine's batches: a unit's
+// level is 0 when no row in it reads a column of another unit, else one more
+// than the largest level among the units it reads.
+//
+// Units are claimed through a ticket in solve order, kLevelClaim rounds of
+// 64 / G units per claim (G lanes per unit): a claimed unit reads only lower
+// units, claimed earlier by running wavefronts, and every wavefront finishes
+// its units in claim order, so the lowest unfinished unit always has its
+// inputs and the sweep completes with any grid size.
+#pragma once
+
+#include "sptrsv_device.hpp"
+
+namespace sptrsv {
+
+struct UnitLevelArgs {
+  const int *__restrict__ rowPtr;  // [n+1] CSR of the factor
+  const int *__restrict__ colIdx;  // [nnz]
+  const int *__restrict__ ufirst;  // [nunits+1] first position of each unit,
+                                   // or null: every position is a unit
+  const int *__restrict__ uid;     // [n] unit of each solve position, or null
+  int *lev;                        // [nunits], -1 until set
+  int n, nunits, backward;
+  int *ticket;                     // [2]: claims, finished workgroups
+  int *err;
+  long long wait_cap;
+};
+
+constexpr int kLevelClaim = 4;  // rounds of 64 / G units per ticket claim
+constexpr int kLevelU = 4;      // entries per lane per poll round
+
+template <int G>
+__global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
+  constexpr int kPer = kWave / G;  // units per wavefront round
+  const int lane = __lane_id();
+  const int gl = lane & (G - 1);
+  const int grp = lane / G;
+  const unsigned long long gmask =
+      (G == kWave ? ~0ull : ((1ull << (G % kWave)) - 1)) << (lane & ~(G - 1));
+  const int n = a.n;
+  StallWatch watch;
+  int level = 0;
+  bool stopped = false;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = stopped ? a.nunits : atomicAdd(a.ticket, kLevelClaim * kPer);
+    base = __shfl(base, 0);
+    if (base >= a.nunits || stopped) break;
+    for (int rd = 0; rd < kLevelClaim && !stopped; rd++) {
+      const int u = base + rd * kPer + grp;
+      const bool act = u < a.nunits;
+      int f = 0, j = 0, hi = 0;
+      if (act) {
+        f = a.ufirst ? a.ufirst[u] : u;
+        const int l = a.ufirst ? a.ufirst[u + 1] : u + 1;
+        const int r0 = a.backward ? n - l : f, r1 = a.backward ? n - f : l;
+        j = a.rowPtr[r0] + gl;
+        hi = a.rowPtr[r1];
+      }
+      bool fin = !act || j >= hi;
+      bool pub = !act;
+      bool chunk = false;
+      unsigned pend = 0;
+      int dep[kLevelU];
+      int m = 0;  // largest level read + 1
+      while (__ballot(!pub) != 0) {
+        if (!fin && !chunk) {
+#pragma unroll
+          for (int q = 0; q < kLevelU; q++) {
+            const int jj = j + q * G;
+            dep[q] = -1;
+            if (jj < hi) {
+              const int c = a.colIdx[jj];
+              const int pc = a.backward ? n - 1 - c : c;
+              if (pc < f) {  // a column of an earlier unit
+                dep[q] = a.uid ? a.uid[pc] : pc;
+                pend |= 1u << q;
+              }
+            }
+          }
+          chunk = true;
+        }
+        unsigned got = 0;
+#pragma unroll
+        for (int q = 0; q < kLevelU; q++)
+          if (pend & (1u << q)) {
+            const int lv = rmw_read(&a.lev[dep[q]]);
+            if (lv >= 0) {
+              m = max(m, lv + 1);
+              got |= 1u << q;
+            }
+          }
+        pend &= ~got;
+        if (chunk && pend == 0) {
+          chunk = false;
+          j += G * kLevelU;
+          fin = j >= hi;
+        }
+        const bool gdone = !pub && (__ballot(fin) & gmask) == gmask;
+        const bool any_done = __ballot(gdone) != 0;
+        if (any_done) {
+          int mm = m;
+#pragma unroll
+          for (int off = G / 2; off > 0; off >>= 1) mm = max(mm, __shfl_xor(mm, off));
+          if (gdone) {
+            if (gl == 0) xchg_agent(&a.lev[u], mm);
+            pub = true;
+          }
+        }
+        const bool any = any_done || __ballot(got != 0) != 0;
+        backoff(level, any);
+        if (watch.step(any, a.err, a.wait_cap)) {
+          stopped = true;
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(a.ticket + 1, 1);
+    if (done == (int)gridDim.x - 1) {
+      xchg_agent(a.ticket, 0);
+      xchg_agent(a.ticket + 1, 0);
+    }
+  }
+}
+
+// Histogram of the levels and the largest level (cnt zeroed, *maxlev = -1).
+__global__ void k_level_histogram(const int *__restrict__ lev, int len, int *cnt,
+                                  int *maxlev) {
+  int mx = -1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += gridDim.x * blockDim.x) {
+    const int v = lev[i];
+    atomicAdd(&cnt[v], 1);
+    mx = max(mx, v);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+  if (__lane_id() == 0) atomicMax(maxlev, mx);
+}
+
+// Level order: off[] holds each level's first place (exclusive scan of the
+// widths) and is advanced as places are taken.
+__global__ void k_level_scatter(const int *__restrict__ lev, int len, int *off,
+                                int *order) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += gridDim.x * blockDim.x)
+    order[atomicAdd(&off[lev[i]], 1)] = i;
+}
+
+}  // namespace sptrsv

@@ -16,12 +16,20 @@
 // fences, no left_sum, and the sum of a row has a fixed order (per lane in
 // entry order, then a fixed butterfly), so the result is deterministic.
 //
-// Work assignment is static: G lanes per (row, right-hand side) item, items
-// in solve order, group g of the grid takes items g, g + ngroups, ... .  An
-// item only depends on items of lower index, every group takes its items in
-// increasing order and every group is resident (the grid is the resident
-// capacity), so the lowest unsolved item always has its inputs and is being
-// worked on: the solve always completes.
+// Work is claimed through a ticket, by workgroups, in the level order of the
+// rows (the analysis sorts the solve positions by DAG level): a claim is
+// `rounds` rounds of one item per group of the workgroup (item = (row,
+// right-hand-side tile), tiles fastest), consecutive items going to
+// different groups.  Items that follow each other in level order are mostly
+// independent, so a workgroup's rounds rarely wait on one another, and the
+// claims are few (one atomic per rounds * groups items).  A claimed item
+// reads only items of lower levels, claimed earlier by running workgroups;
+// every wavefront finishes its items of a claim in order and a workgroup
+// claims again only when all its wavefronts are done, so the lowest
+// unfinished item always has its inputs and is being worked on: the solve
+// completes with any grid size, whether or not every workgroup is resident
+// (the reference's dynamic column ticket, sptrsv_syncfree_cuda.h:128-131,
+// plays the same role in row order).
 #pragma once
 
 #include "sptrsv_device.hpp"
@@ -32,16 +40,19 @@ namespace sptrsv {
 template <typename T>
 struct RowArgs {
   const int *__restrict__ rowPtr;  // [n+1] CSR of the triangle
+  const int *__restrict__ order;   // [n] solve positions in level order
   const int *__restrict__ colIdx;  // [nnz] columns ascending in each row
   const T *__restrict__ val;       // [nnz]
   const T *__restrict__ b;         // [n][rhs]
   T *x;                            // [n][rhs], the sentinel until solved
-  int n, rhs;
+  int n, rhs, ntiles;              // items: n * ntiles
+  int rounds;                      // rounds per workgroup claim
+  int *ticket;                     // [2]: claims, finished workgroups
   int *err;
   long long wait_cap;
 };
 
-constexpr int kRowU = 4;  // entries per lane per poll round
+constexpr int kRowU = 4;      // entries per lane per poll round
 
 // Read a location other workgroups write with an exchange, through an RMW
 // that leaves it unchanged (performed where every XCD sees the latest value).
@@ -58,113 +69,185 @@ __device__ __forceinline__ float rmw_read_val(float *p) {
   return __builtin_bit_cast(float, v);
 }
 
-// G lanes per item (G a power of two, at most 64).  Forward: the factor is
-// lower triangular, the diagonal is the last entry of its row; backward:
-// upper triangular, the diagonal is the first entry.  The groups of a
-// wavefront advance independently (an item may depend on an item of another
-// group of the same wavefront): each lane loads kRowU of its entries, polls
-// them until all are in, accumulates, and moves on; a group whose lanes have
-// all run out of entries reduces and publishes at once.
-template <typename T, bool BWD, int G>
+// G lanes per item: RL of them over the tile's right-hand sides (RPL values
+// each, tile width RL * RPL), NS = G / RL over the row's entries.  RL = 1 is
+// the per-(row, right-hand side) pull of a single solution; RL = G reads whole
+// rows of x ([n][rhs] row-major, so a tile is one contiguous segment) and
+// passes over the factor once for the whole tile.
+//
+// Forward: the factor is lower triangular, the diagonal is the last entry of
+// its row; backward: upper triangular, the diagonal is the first entry.  The
+// groups of a wavefront advance independently (an item may depend on an item
+// of another group of the same wavefront): each lane loads kRowU of its
+// entries, polls them until all are in, accumulates, and moves on; a group
+// whose lanes have all run out of entries reduces and publishes at once.
+// Polls and publishes: with one value per item (RL = 1) an RMW that leaves
+// the value unchanged and an exchange; with whole rows of x, agent-scope
+// atomic loads and stores (coalesced, and a right-hand-side tile of a row is
+// hundreds of values: RMWs would serialise on the atomic units).
+template <typename T, bool BWD, int G, int RL, int RPL>
 __global__ void __launch_bounds__(1024) k_exec_row(RowArgs<T> a) {
+  constexpr int NS = G / RL;  // lanes over entries
+  constexpr bool kBulk = RL > 1;
+  constexpr int kPer = kWave / G;
   const int lane = __lane_id();
   const int gl = lane & (G - 1);
+  const int lr = gl % RL;   // right-hand-side lane
+  const int ls = gl / RL;   // entry lane
   const unsigned long long gmask =
-      (G == kWave ? ~0ull : ((1ull << G) - 1)) << (lane & ~(G - 1));
-  const int waves_per_block = blockDim.x / kWave;
-  const int gwave = blockIdx.x * waves_per_block + threadIdx.x / kWave;
-  const long long ngroups = (long long)gridDim.x * waves_per_block * (kWave / G);
-  const long long gidx = (long long)gwave * (kWave / G) + lane / G;
-  const long long items = (long long)a.n * a.rhs;
+      (G == kWave ? ~0ull : ((1ull << (G % kWave)) - 1)) << (lane & ~(G - 1));
+  const long long items = (long long)a.n * a.ntiles;
+  const int rhs = a.rhs;
+  const int gpb = (blockDim.x / kWave) * kPer;  // groups per workgroup
+  const int chunk = a.rounds * gpb;             // items per claim
+  const int grp = (threadIdx.x / kWave) * kPer + lane / G;
+  __shared__ long long s_claim;
   StallWatch watch;
   int level = 0;
   bool stopped = false;
-  // wave-uniform rounds: in round m the groups take items m * ngroups + gidx
-  for (long long base = 0; base < items && !stopped; base += ngroups) {
-    const long long t = base + gidx;
-    const bool act = t < items;
-    int r = 0, q = 0, j = 0, hi = 0;
-    T d = T(1), bv = T(0);
-    if (act) {
-      const int p = (int)(t / a.rhs);
-      q = (int)(t - (long long)p * a.rhs);
-      r = BWD ? a.n - 1 - p : p;
-      const int s0 = a.rowPtr[r], s1 = a.rowPtr[r + 1];
-      d = a.val[BWD ? s0 : s1 - 1];
-      bv = a.b[(size_t)r * a.rhs + q];
-      j = (BWD ? s0 + 1 : s0) + gl;
-      hi = BWD ? s1 : s1 - 1;
+  if (threadIdx.x == 0) s_claim = atomicAdd(a.ticket, chunk);
+  __syncthreads();
+  for (;;) {
+    const long long base = s_claim;
+    __syncthreads();  // every wavefront has read the claim
+    if (base >= items) break;
+    // the next claim is in flight while this one is worked on (after an
+    // error every wavefront's stall watch stops it; the remaining claims
+    // then pass through without work)
+    if (threadIdx.x == 0) s_claim = stopped ? items : (long long)atomicAdd(a.ticket, chunk);
+    for (int rd = 0; rd < a.rounds && !stopped; rd++) {
+      const long long t = base + (long long)rd * gpb + grp;
+      const bool act = t < items;
+      int r = 0, tile = 0, j = 0, hi = 0;
+      T d = T(1);
+      T bv[RPL];
+      bool qv[RPL];
+#pragma unroll
+      for (int q = 0; q < RPL; q++) {
+        bv[q] = T(0);
+        qv[q] = false;
+      }
+      if (act) {
+        const int idx = (int)(t / a.ntiles);
+        tile = (int)(t - (long long)idx * a.ntiles);
+        const int p = a.order[idx];
+        r = BWD ? a.n - 1 - p : p;
+        const int s0 = a.rowPtr[r], s1 = a.rowPtr[r + 1];
+        d = a.val[BWD ? s0 : s1 - 1];
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+          const int k = tile * (RL * RPL) + q * RL + lr;
+          qv[q] = k < rhs;
+          if (ls == 0 && qv[q]) bv[q] = a.b[(size_t)r * rhs + k];
+        }
+        j = (BWD ? s0 + 1 : s0) + ls;
+        hi = BWD ? s1 : s1 - 1;
+      }
+      bool fin = !act || j >= hi;  // this lane has no entries left
+      bool pub = !act;             // the group's item is published
+      bool chunk = false;          // entries loaded, not yet accumulated
+      unsigned pend = 0;           // bit u * RPL + q: value still awaited
+      int col[kRowU];
+      T v[kRowU], xv[kRowU][RPL];
+      T acc[RPL];
+#pragma unroll
+      for (int q = 0; q < RPL; q++) acc[q] = T(0);
+      while (__ballot(!pub) != 0) {
+        if (!fin && !chunk) {
+#pragma unroll
+          for (int u = 0; u < kRowU; u++) {
+            const int jj = j + u * NS;
+            const bool has = jj < hi;
+            col[u] = has ? a.colIdx[jj] : 0;
+            v[u] = has ? a.val[jj] : T(0);
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+              xv[u][q] = T(0);
+              if (has && qv[q]) pend |= 1u << (u * RPL + q);
+            }
+          }
+          chunk = true;
+        }
+        unsigned got = 0;
+#pragma unroll
+        for (int u = 0; u < kRowU; u++)
+#pragma unroll
+          for (int q = 0; q < RPL; q++)
+            if (pend & (1u << (u * RPL + q))) {
+              T *px = &a.x[(size_t)col[u] * rhs + tile * (RL * RPL) + q * RL + lr];
+              const T y = kBulk ? ld_agent(px) : rmw_read_val(px);
+              if (!slot_empty(y)) {
+                xv[u][q] = y;
+                got |= 1u << (u * RPL + q);
+              }
+            }
+        pend &= ~got;
+        if (chunk && pend == 0) {
+#pragma unroll
+          for (int u = 0; u < kRowU; u++)
+#pragma unroll
+            for (int q = 0; q < RPL; q++) acc[q] = __builtin_fma(v[u], xv[u][q], acc[q]);
+          chunk = false;
+          j += NS * kRowU;
+          fin = j >= hi;
+        }
+        // groups whose lanes all ran out of entries reduce and publish
+        const bool gdone = !pub && (__ballot(fin) & gmask) == gmask;
+        const bool any_done = __ballot(gdone) != 0;
+        if (any_done) {
+          // fixed-order butterfly over the entry lanes (every lane takes
+          // part; only finished groups use the sums, the others keep
+          // accumulating into acc)
+          T sum[RPL];
+#pragma unroll
+          for (int q = 0; q < RPL; q++) {
+            T s = acc[q];
+#pragma unroll
+            for (int off = G / 2; off >= RL; off >>= 1) s += __shfl_xor(s, off);
+            sum[q] = s;
+          }
+          if (gdone) {
+            if (ls == 0) {
+              const T inv = T(1) / d;
+#pragma unroll
+              for (int q = 0; q < RPL; q++)
+                if (qv[q]) {
+                  const T rr = bv[q] - sum[q];
+                  T xr = rr * inv;
+                  xr = __builtin_fma(__builtin_fma(-xr, d, rr), inv, xr);
+                  T *px = &a.x[(size_t)r * rhs + tile * (RL * RPL) + q * RL + lr];
+                  if (kBulk)
+                    st_agent(px, xr);
+                  else
+                    xchg_agent(px, xr);
+                }
+            }
+            pub = true;
+          }
+        }
+        // a wavefront per row: a short fixed pause (its inputs are usually one
+        // publish away; 8% faster on the 10^6 power-law factor); narrower
+        // groups, many rows per wavefront: exponential back-off (a fixed pause
+        // made the 700^2 grid 30% slower)
+        const bool any = any_done || __ballot(got != 0) != 0;
+        if constexpr (G == kWave) {
+          if (!any) __builtin_amdgcn_s_sleep(1);
+        } else {
+          backoff(level, any);
+        }
+        if (watch.step(any, a.err, a.wait_cap)) {
+          stopped = true;
+          break;
+        }
+      }
     }
-    bool fin = !act || j >= hi;  // this lane has no entries left
-    bool pub = !act;             // the group's item is published
-    bool chunk = false;          // entries loaded, not yet accumulated
-    unsigned pend = 0;
-    int col[kRowU];
-    T v[kRowU], xv[kRowU];
-    T acc = T(0);
-    while (__ballot(!pub) != 0) {
-      if (!fin && !chunk) {
-#pragma unroll
-        for (int u = 0; u < kRowU; u++) {
-          const int jj = j + u * G;
-          const bool has = jj < hi;
-          col[u] = has ? a.colIdx[jj] : 0;
-          v[u] = has ? a.val[jj] : T(0);
-          xv[u] = T(0);
-          if (has) pend |= 1u << u;
-        }
-        chunk = true;
-      }
-      unsigned got = 0;
-#pragma unroll
-      for (int u = 0; u < kRowU; u++)
-        if (pend & (1u << u)) {
-          const T y = rmw_read_val(&a.x[(size_t)col[u] * a.rhs + q]);
-          if (!slot_empty(y)) {
-            xv[u] = y;
-            got |= 1u << u;
-          }
-        }
-      pend &= ~got;
-      if (chunk && pend == 0) {
-#pragma unroll
-        for (int u = 0; u < kRowU; u++) acc = __builtin_fma(v[u], xv[u], acc);
-        chunk = false;
-        j += G * kRowU;
-        fin = j >= hi;
-      }
-      // groups whose lanes all ran out of entries reduce and publish
-      const bool gdone = !pub && (__ballot(fin) & gmask) == gmask;
-      const bool any_done = __ballot(gdone) != 0;
-      if (any_done) {
-        T s = acc;  // fixed-order butterfly over the group's lanes
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (gdone) {
-          if (gl == 0) {
-            const T rr = bv - s;
-            const T inv = T(1) / d;
-            T xr = rr * inv;
-            xr = __builtin_fma(__builtin_fma(-xr, d, rr), inv, xr);
-            xchg_agent(&a.x[(size_t)r * a.rhs + q], xr);
-          }
-          pub = true;
-        }
-      }
-      // a wavefront per row: a short fixed pause (its inputs are usually one
-      // publish away; 8% faster on the 10^6 power-law factor); narrower
-      // groups, many rows per wavefront: exponential back-off (a fixed pause
-      // made the 700^2 grid 30% slower)
-      const bool any = any_done || __ballot(got != 0) != 0;
-      if constexpr (G == kWave) {
-        if (!any) __builtin_amdgcn_s_sleep(1);
-      } else {
-        backoff(level, any);
-      }
-      if (watch.step(any, a.err, a.wait_cap)) {
-        stopped = true;
-        break;
-      }
+    __syncthreads();  // the claim is done; s_claim holds the next one
+  }
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(a.ticket + 1, 1);
+    if (done == (int)gridDim.x - 1) {
+      xchg_agent(a.ticket, 0);
+      xchg_agent(a.ticket + 1, 0);
     }
   }
 }

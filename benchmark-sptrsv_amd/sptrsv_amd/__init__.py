@@ -7,7 +7,10 @@ substitution, rhs, opt, x (overwritten), b, x_ref; it returns
 device-resident arrays.  Everything runs in libsptrsv_hip.so; nothing here
 computes a solve.
 """
+import atexit
 import ctypes
+import sys
+import weakref
 
 import numpy as np
 
@@ -34,6 +37,31 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+# Live library objects, released at interpreter exit in dependency order
+# (graphs, plans, events, streams, buffers) while the HIP runtime is still up;
+# finalizers that run later, during interpreter teardown, do nothing.
+_LIVE = {k: weakref.WeakSet() for k in ("graph", "plan", "event", "stream", "array")}
+
+
+@atexit.register
+def _release_all():
+    for kind in ("graph", "plan", "event", "stream", "array"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj._release()
+            except Exception:
+                pass
+
+
+def _finalize(obj):
+    if sys.is_finalizing():
+        return
+    try:
+        obj._release()
+    except Exception:
+        pass
+
+
 def device_count():
     n = ctypes.c_int(0)
     st = lib().sptrsv_device_count(ctypes.byref(n))
@@ -56,6 +84,7 @@ class DeviceArray:
         p = ctypes.c_void_p()
         check(lib().sptrsv_malloc(ctypes.byref(p), self.nbytes), "sptrsv_malloc")
         self.ptr = p
+        _LIVE["array"].add(self)
 
     @classmethod
     def from_host(cls, arr):
@@ -86,11 +115,10 @@ class DeviceArray:
             lib().sptrsv_free(self.ptr)
             self.ptr = None
 
+    _release = free
+
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Stream:
@@ -98,16 +126,18 @@ class Stream:
         p = ctypes.c_void_p()
         check(lib().sptrsv_stream_create(ctypes.byref(p)), "stream_create")
         self.ptr = p
+        _LIVE["stream"].add(self)
 
     def synchronize(self):
         check(lib().sptrsv_stream_synchronize(self.ptr), "stream_synchronize")
 
+    def _release(self):
+        if self.ptr:
+            lib().sptrsv_stream_destroy(self.ptr)
+            self.ptr = None
+
     def __del__(self):
-        try:
-            if self.ptr:
-                lib().sptrsv_stream_destroy(self.ptr)
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Event:
@@ -115,6 +145,7 @@ class Event:
         p = ctypes.c_void_p()
         check(lib().sptrsv_event_create(ctypes.byref(p)), "event_create")
         self.ptr = p
+        _LIVE["event"].add(self)
 
     def record(self, stream=None):
         check(lib().sptrsv_event_record(self.ptr, stream.ptr if stream else None), "event_record")
@@ -124,12 +155,13 @@ class Event:
         check(lib().sptrsv_event_elapsed(self.ptr, later.ptr, ctypes.byref(ms)), "event_elapsed")
         return ms.value
 
+    def _release(self):
+        if self.ptr:
+            lib().sptrsv_event_destroy(self.ptr)
+            self.ptr = None
+
     def __del__(self):
-        try:
-            if self.ptr:
-                lib().sptrsv_event_destroy(self.ptr)
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def _sp(stream):
@@ -158,6 +190,7 @@ class Plan:
                                                 substitution, flags, _sp(stream))
         check(st, "plan_create")
         self.ptr = p
+        _LIVE["plan"].add(self)
 
     @classmethod
     def from_host(cls, colPtr, rowIdx, val, substitution=SUBSTITUTION_FORWARD,
@@ -206,19 +239,29 @@ class Plan:
         check(lib().sptrsv_plan_info(self.ptr, ctypes.byref(i)), "plan_info")
         return {f: getattr(i, f) for f, _ in i._fields_}
 
+    def prepare(self, rhs=1, mode=0, stream=None):
+        """Run the per-configuration analysis now (instead of at the first solve)."""
+        check(lib().sptrsv_plan_prepare(self.ptr, rhs, mode, _sp(stream)), "prepare")
+
+    def levels(self, stream=None):
+        """(nlevel, min, avg, max) rows per level, computed on the device."""
+        out = [ctypes.c_int() for _ in range(4)]
+        check(lib().sptrsv_plan_levels(self.ptr, *[ctypes.byref(o) for o in out],
+                                       _sp(stream)), "plan_levels")
+        return tuple(o.value for o in out)
+
     def graph(self, b, x, rhs=1, mode=0, iters=0, stream=None):
         return Graph(self, b, x, rhs, mode, iters, stream)
 
     def destroy(self):
         if getattr(self, "ptr", None):
-            lib().sptrsv_plan_destroy(self.ptr)
+            check(lib().sptrsv_plan_destroy(self.ptr), "plan_destroy")
             self.ptr = None
 
+    _release = destroy
+
     def __del__(self):
-        try:
-            self.destroy()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Graph:
@@ -228,16 +271,20 @@ class Graph:
                                         iters, _sp(stream)), "graph_create")
         self.ptr = p
         self._keep = (plan, b, x)
+        _LIVE["graph"].add(self)
 
     def launch(self, stream=None):
         check(lib().sptrsv_graph_launch(self.ptr, _sp(stream)), "graph_launch")
 
+    def destroy(self):
+        if getattr(self, "ptr", None):
+            lib().sptrsv_graph_destroy(self.ptr)
+            self.ptr = None
+
+    _release = destroy
+
     def __del__(self):
-        try:
-            if self.ptr:
-                lib().sptrsv_graph_destroy(self.ptr)
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def _dropin(fn, colPtr, rowIdx, val, m, n, nnz, substitution, rhs, opt, x, b, x_ref, vdt):

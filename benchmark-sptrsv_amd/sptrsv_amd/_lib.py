@@ -60,6 +60,7 @@ EXPORTS = [
     "sptrsv_plan_solve_ir", "sptrsv_plan_autotune", "sptrsv_graph_create",
     "sptrsv_graph_launch", "sptrsv_graph_destroy", "sptrsv_plan_check",
     "sptrsv_plan_reset", "sptrsv_plan_set_spin_cap", "sptrsv_plan_info",
+    "sptrsv_plan_prepare", "sptrsv_plan_levels",
 ]
 
 
@@ -95,6 +96,7 @@ class PlanInfo(ctypes.Structure):
         ("levels", ctypes.c_int),
         ("scratch_bytes", ctypes.c_longlong),
         ("analysis_ms", ctypes.c_double),
+        ("prepare_ms", ctypes.c_double),
     ]
 
 
@@ -162,6 +164,8 @@ def _sig(lib):
     s("sptrsv_plan_reset", [P, P])
     s("sptrsv_plan_set_spin_cap", [P, ctypes.c_longlong])
     s("sptrsv_plan_info", [P, ctypes.POINTER(PlanInfo)])
+    s("sptrsv_plan_prepare", [P, I, I, P])
+    s("sptrsv_plan_levels", [P, IP, IP, IP, IP, P])
 
 
 def lib():

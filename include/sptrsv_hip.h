@@ -19,6 +19,9 @@
  *                                   sptrsv_syncfree_cuda, sptrsv_syncfree_cuda.h:456-543
  *   sptrsv_equal_*_partition     <- equal_col_partition / equal_nnz_partition, utils.h:7-67
  *   sptrsv_levels                <- findlevel_csc, SpTRSV_cuda/findlevel.h:71-147
+ *   sptrsv_plan_levels           <- the same statistics, computed on the device
+ *   sptrsv_plan_prepare          <- the analyser step of sptrsv_syncfree_cuda,
+ *                                   sptrsv_syncfree_cuda.h:456-483
  */
 #ifndef SPTRSV_HIP_H
 #define SPTRSV_HIP_H
@@ -245,6 +248,20 @@ int sptrsv_plan_get_config(sptrsv_plan_t plan, int rhs, sptrsv_config_t *cfg);
 int sptrsv_plan_solve(sptrsv_plan_t plan, const void *d_b, void *d_x, int rhs,
                       void *stream);
 
+/* Run the per-configuration analysis that the first solve with `rhs`
+ * right-hand sides would run (mode 0 = plain solve, 1 = the FP32 solve of IR):
+ * window set, lane targets, batches and their levels, receive lists, the CSR
+ * copy of the row engine.  Lets a caller keep it out of timed or captured
+ * regions (the reference times its analyser apart from the solves,
+ * sptrsv_syncfree_cuda.h:456-483).  Synchronous. */
+int sptrsv_plan_prepare(sptrsv_plan_t plan, int rhs, int mode, void *stream);
+
+/* DAG depth and level widths of the plan's factor, computed on the device
+ * (the statistics of findlevel_csc, findlevel.h:71-147: number of levels,
+ * min / average / max rows per level).  Cached; synchronous. */
+int sptrsv_plan_levels(sptrsv_plan_t plan, int *nlevel, int *pmin, int *pavg,
+                       int *pmax, void *stream);
+
 /* FP32 solves + FP64 residual/update iterative refinement (plan created with
  * SPTRSV_PLAN_MIXED on FP64 data).  iters refinement steps after the first
  * FP32 solve.  d_b, d_x are FP64. */
@@ -278,9 +295,13 @@ int sptrsv_plan_set_spin_cap(sptrsv_plan_t plan, long long cap_us);
 
 typedef struct {
   int n, nnz, substitution, value_type, nsys;
-  int levels;                /* DAG depth (0 if not computed)          */
+  int levels;                /* DAG depth (0 until sptrsv_plan_levels or
+                                sptrsv_plan_autotune computed it)       */
   long long scratch_bytes;   /* device scratch held by the plan        */
   double analysis_ms;        /* device analysis time at creation       */
+  double prepare_ms;         /* per-configuration analyses so far (the
+                                first solve of each configuration, or
+                                sptrsv_plan_prepare), host wall clock  */
 } sptrsv_plan_info_t;
 int sptrsv_plan_info(sptrsv_plan_t plan, sptrsv_plan_info_t *info);
 

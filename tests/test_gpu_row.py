@@ -6,6 +6,8 @@ the serial oracle, refinement 1e-10 against the FP64 oracle.  The row sums
 have a fixed order (per lane in entry order, then a fixed butterfly), so two
 solves of the same system are bit-identical, which is checked too.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -149,7 +151,7 @@ def test_row_engine_config_checks():
             plan.set_config(1, **bad)
     plan.set_config(1, engine=ROW)
     c = plan.get_config(1)
-    assert c["engine"] == ROW and c["lanes_per_column"] == 64 and c["block"] == 256
+    assert c["engine"] == ROW and c["lanes_per_column"] == 64 and c["block"] == 1024
 
 
 def test_default_engine_follows_entry_distance():
@@ -167,3 +169,83 @@ def test_default_engine_follows_entry_distance():
         b = f.spmv(mx.x_uniform(f.n, seed=4))
         x, _ = plan_solve(f, b, plan=plan)
         assert_close(x, oracle_x(f, b), 1e-12)
+
+
+# ------------------------------------------------ lanes over right-hand sides
+# (lanes per row, lanes over right-hand sides, right-hand sides per lane):
+# one pass over the factor for a tile of RL * RPL right-hand sides, whole rows
+# of x read and written at once; rhs counts that are not a multiple of the
+# tile leave lanes idle in the last tile.
+RHS_LAYOUTS = [(8, 8, 1), (16, 16, 1), (32, 32, 1), (64, 64, 1), (64, 64, 2), (64, 64, 4)]
+
+
+def _rl_id(t):
+    return "G{}-RL{}-RPL{}".format(*t)
+
+
+@pytest.mark.parametrize("layout", RHS_LAYOUTS, ids=_rl_id)
+@pytest.mark.parametrize("rhs", [3, 8, 40, 130])
+def test_row_engine_rhs_lanes(layout, rhs):
+    G, RL, RPL = layout
+    f = mx.laplacian3d_lower(14, 12, 10)
+    X = mx.x_normal(f.n, rhs, seed=rhs)
+    B = f.spmv(X)
+    cfg = dict(engine=ROW, lanes_per_column=G, rhs_lanes=RL, rhs_per_lane=RPL, block=256)
+    x, plan = plan_solve(f, B, rhs=rhs, cfg=cfg)
+    assert_close(x, oracle_x(f, B, rhs=rhs), 1e-12)
+    x2, _ = plan_solve(f, B, rhs=rhs, plan=plan)
+    assert np.array_equal(x, x2)  # fixed summation order
+
+
+@pytest.mark.parametrize("layout", [(16, 16, 1), (64, 64, 2)], ids=_rl_id)
+def test_row_engine_rhs_lanes_powerlaw_backward_fp32(layout):
+    G, RL, RPL = layout
+    cfg = dict(engine=ROW, lanes_per_column=G, rhs_lanes=RL, rhs_per_lane=RPL, block=1024)
+    f = mx.power_law_lower(12_000, window=1500, seed=13)
+    X = mx.x_normal(f.n, 33, seed=14)
+    B = f.spmv(X)
+    x, _ = plan_solve(f, B, rhs=33, cfg=cfg)
+    assert_close(x, oracle_x(f, B, rhs=33), 1e-12)
+    u = mx.upper_from_lower(f)
+    Bu = u.spmv(X)
+    x, _ = plan_solve(u, Bu, rhs=33, subst=1, cfg=cfg)
+    assert_close(x, oracle_x(u, Bu, rhs=33, subst=1), 1e-12)
+    x, _ = plan_solve(f, B, rhs=33, cfg=cfg, dtype=np.float32)
+    assert_close(x, oracle_x(f, B, rhs=33, dtype=np.float32), 2e-5)
+
+
+def test_row_engine_rhs_lanes_refinement():
+    f = mx.laplacian3d_lower(24, 20, 18)
+    X = mx.x_normal(f.n, 64, seed=15)
+    B = f.spmv(X)
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val, flags=S.PLAN_VALIDATE | S.PLAN_MIXED)
+    plan.set_config(64, engine=ROW, lanes_per_column=64, rhs_lanes=64, rhs_per_lane=1,
+                    block=512)
+    db = S.DeviceArray.from_host(B)
+    dx = S.DeviceArray(B.shape, np.float64)
+    plan.solve_ir(db, dx, rhs=64, iters=2)
+    assert plan.check() == 0
+    assert_close(dx.to_host(), oracle_x(f, B, rhs=64), 1e-10)
+
+
+# ------------------------------------------------ ticket: any grid size works
+@pytest.mark.parametrize("cfg", CFGS + [dict(engine=ROW, lanes_per_column=64, rhs_lanes=64,
+                                             rhs_per_lane=2, block=256)],
+                         ids=lambda c: "G{lanes_per_column}-B{block}".format(**c)
+                         + ("-RL" if c.get("rhs_lanes", 1) > 1 else ""))
+def test_row_engine_grid_beyond_resident_capacity(cfg):
+    # four times as many workgroups as can be resident: items are claimed in
+    # solve order through the ticket, so workgroups that start late only take
+    # later items; a short spin cap turns any wait on a missing producer into
+    # a bounded error instead of a hang
+    cus = ctypes.c_int(0)
+    S.check(S.lib().sptrsv_device_cu_count(ctypes.byref(cus)), "cu_count")
+    per_cu = 2048 // cfg["block"] * 4  # at most 2048 threads per CU
+    rhs = 4 if cfg.get("rhs_lanes", 1) == 1 else 128
+    f = mx.power_law_lower(30_000, window=3000, seed=16)
+    X = mx.x_normal(f.n, rhs, seed=17)
+    B = f.spmv(X)
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
+    plan.set_spin_cap(2_000_000)  # 2 s
+    x, _ = plan_solve(f, B, rhs=rhs, cfg=dict(cfg, grid=cus.value * per_cu), plan=plan)
+    assert_close(x, oracle_x(f, B, rhs=rhs), 1e-12)
