@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1971,6 +1972,77 @@ int sptrsv_event_elapsed(void *a, void *b, float *ms) {
 int sptrsv_exclusive_scan(int *d, int len, void *stream) {
   if (len < 0 || (len > 0 && !d)) return SPTRSV_ERR_INVALID_ARG;
   return scan_rec(d, len, as_stream(stream));
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch of the allocation-free scan: the tile totals of every level of
+// the recursive scan (level 1: ceil(len / kScanTile) ints, level 2:
+// ceil(level 1 / kScanTile), ...), for len up to SPTRSV_SCAN_NOALLOC_MAX.
+constexpr int kNoAllocL1 = (SPTRSV_SCAN_NOALLOC_MAX + kScanTile - 1) / kScanTile;
+constexpr int kNoAllocL2 = (kNoAllocL1 + kScanTile - 1) / kScanTile;
+constexpr int kNoAllocInts = kNoAllocL1 + kNoAllocL2 + 64;
+static_assert(kNoAllocL2 <= kScanTile, "three scan levels cover the maximum length");
+
+// One copy per device, held until the process ends.  In host code the name
+// is only a stand-in for the device symbol: kernels get the address returned
+// by hipGetSymbolAddress for the current device.
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+constexpr int kMaxDevices = 64;
+int *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The current device's address of `scratch`, looked up once per device.
+int scratch_address(int **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return SPTRSV_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return SPTRSV_ERR_HIP;
+    g_scratch_ptr[dev] = reinterpret_cast<int *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return SPTRSV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sptrsv_exclusive_scan_noalloc(int *d, int len, void *stream) {
+  if (len < 0 || (len > 0 && !d)) return SPTRSV_ERR_INVALID_ARG;
+  if (len > SPTRSV_SCAN_NOALLOC_MAX) return SPTRSV_ERR_INVALID_ARG;
+  if (len == 0) return SPTRSV_OK;
+  int *sc = nullptr;
+  ST_TRY(scratch_address(&sc));
+  hipStream_t s = as_stream(stream);
+  // the same kernels as sptrsv_exclusive_scan, with the tile totals of each
+  // level in the static scratch instead of per-call allocations; nothing is
+  // freed, so the stream is not synchronised
+  int *l1 = sc, *l2 = sc + kNoAllocL1, *l3 = sc + kNoAllocL1 + kNoAllocL2;
+  const int n1 = (len + kScanTile - 1) / kScanTile;
+  const int n2 = (n1 + kScanTile - 1) / kScanTile;
+  k_scan_tile<<<n1, kScanThreads, 0, s>>>(d, len, l1);
+  HIP_TRY(hipGetLastError());
+  if (n1 > 1) {
+    k_scan_tile<<<n2, kScanThreads, 0, s>>>(l1, n1, l2);
+    HIP_TRY(hipGetLastError());
+    if (n2 > 1) {
+      k_scan_tile<<<1, kScanThreads, 0, s>>>(l2, n2, l3);
+      HIP_TRY(hipGetLastError());
+      k_scan_add<<<n2, kScanThreads, 0, s>>>(l1, n1, l2);
+      HIP_TRY(hipGetLastError());
+    }
+    k_scan_add<<<n1, kScanThreads, 0, s>>>(d, len, l1);
+    HIP_TRY(hipGetLastError());
+  }
+  return SPTRSV_OK;
 }
 
 int sptrsv_csr2csc(int m, int n, int nnz, const int *rowPtr, const int *colIdx,

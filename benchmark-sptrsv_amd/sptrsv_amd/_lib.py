@@ -60,7 +60,7 @@ EXPORTS = [
     "sptrsv_plan_solve_ir", "sptrsv_plan_autotune", "sptrsv_graph_create",
     "sptrsv_graph_launch", "sptrsv_graph_destroy", "sptrsv_plan_check",
     "sptrsv_plan_reset", "sptrsv_plan_set_spin_cap", "sptrsv_plan_info",
-    "sptrsv_plan_prepare", "sptrsv_plan_levels",
+    "sptrsv_plan_prepare", "sptrsv_plan_levels", "sptrsv_exclusive_scan_noalloc",
 ]
 
 
@@ -143,6 +143,7 @@ def _sig(lib):
     s("sptrsv_event_record", [P, P])
     s("sptrsv_event_elapsed", [P, P, FP])
     s("sptrsv_exclusive_scan", [P, I, P])
+    s("sptrsv_exclusive_scan_noalloc", [P, I, P])
     s("sptrsv_csr2csc", [I, I, I, P, P, P, I, P, P, P, P])
     s("sptrsv_analyse", [P, I, I, P, P])
     s("sptrsv_validate", [P, P, P, I, I, I, I, P])

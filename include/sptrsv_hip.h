@@ -137,6 +137,17 @@ int sptrsv_event_elapsed(void *start, void *stop, float *ms);
 /* In-place exclusive scan of len ints (utils.h:374-391). */
 int sptrsv_exclusive_scan(int *d_data, int len, void *stream);
 
+/* The same scan with no allocation at all.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Its tile
+ * totals live in a fixed static device array (about 0.5 MB per device, held
+ * until the process ends), so len is limited to SPTRSV_SCAN_NOALLOC_MAX
+ * (larger requests return SPTRSV_ERR_INVALID_ARG).  Asynchronous on the
+ * stream.  All calls on one device share that array: calls that overlap in
+ * time on one device race. */
+#define SPTRSV_SCAN_NOALLOC_MAX (1 << 28)
+int sptrsv_exclusive_scan_noalloc(int *d_data, int len, void *stream);
+
 /* Stable CSR -> CSC transpose of an m x n matrix with nnz entries
  * (tranpose.h:6-43): output columns list their rows in input order, so for a
  * CSR with ascending rows every column comes out ascending.  value_type is
