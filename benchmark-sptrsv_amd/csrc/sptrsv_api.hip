@@ -11,6 +11,7 @@
 #include <chrono>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -781,6 +782,22 @@ T slot_empty_host() {
 
 int ensure_csr(sptrsv_plan_s *p, hipStream_t s);
 
+// Phase times of the per-configuration analysis, printed to stderr at
+// verbose level 2 (each mark synchronises the stream only then).
+struct PhaseTimer {
+  hipStream_t s;
+  std::chrono::steady_clock::time_point t;
+  explicit PhaseTimer(hipStream_t st) : s(st), t(std::chrono::steady_clock::now()) {}
+  void mark(const char *what) {
+    if (g_verbose < 2) return;
+    hipStreamSynchronize(s);
+    const auto now = std::chrono::steady_clock::now();
+    fprintf(stderr, "sptrsv analysis: %-28s %8.3f ms\n", what,
+            std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  }
+};
+
 // Levels of the unit DAG (units = runs of consecutive solve positions given
 // by d_ufirst[nunits+1] and d_uid[n], or single positions when both are null)
 // into d_lev[nunits], by the device pull sweep of k_unit_levels over the
@@ -790,7 +807,6 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
   if (nunits <= 0) return SPTRSV_OK;
   ST_TRY(ensure_csr(p, s));
   ST_TRY(fill_async<int>(d_lev, (size_t)nunits, -1, s));
-  constexpr int G = 16;  // rows have few entries; batches loop over theirs
   UnitLevelArgs a;
   a.rowPtr = p->csrPtr.as<int>();
   a.colIdx = p->csrIdx.as<int>();
@@ -804,10 +820,19 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
   a.err = p->sync.as<int>() + kErrWord;
   // analysis is not subject to the solve's spin cap
   a.wait_cap = kDefaultWaitCapUs * p->clock_khz / 1000;
-  const long long claims =
-      ((long long)nunits + kLevelClaim * (kWave / G) - 1) / (kLevelClaim * (kWave / G));
-  const int grid = (int)std::max(1LL, std::min<long long>((claims + 3) / 4, (long long)p->cus * 8));
-  k_unit_levels<G><<<grid, 256, 0, s>>>(a);
+  // rows (few entries each): 16 lanes per row, 4 rounds of 4 rows per
+  // wavefront claim; batches (up to hundreds of entries): a wavefront per
+  // batch, one batch per claim
+  const int G = d_ufirst ? 64 : 16;
+  a.rounds = d_ufirst ? 1 : 4;
+  const long long per_claim = (long long)a.rounds * (kWave / G);
+  const long long claims = (nunits + per_claim - 1) / per_claim;
+  const int grid =
+      (int)std::max(1LL, std::min<long long>((claims + 3) / 4, (long long)p->cus * 8));
+  if (G == 64)
+    k_unit_levels<64><<<grid, 256, 0, s>>>(a);
+  else
+    k_unit_levels<16><<<grid, 256, 0, s>>>(a);
   HIP_TRY(hipGetLastError());
   ST_TRY(sptrsv_plan_check(p, s));
   return SPTRSV_OK;
@@ -896,6 +921,7 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
                   std::vector<int> *boff_h, std::vector<int> *bord_h,
                   hipStream_t s) {
   const int n = p->n;
+  PhaseTimer pt(s);
   std::vector<int> wp(ws->nwin + 1, 0);
   std::vector<unsigned char> link(std::max(n, 1), 0);
   HIP_TRY(hipMemcpyAsync(wp.data(), ws->winPtr.p, sizeof(int) * wp.size(),
@@ -921,11 +947,14 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   } else {
     HIP_TRY(hipStreamSynchronize(s));
   }
-  std::vector<int> local, first, owner(std::max(n, 1), 0), off(ws->nwin + 1, 0);
-  for (int w = 0; w < ws->nwin; w++) {
+  pt.mark("batches: links");
+  // each window's batches (window-local starts), dealt to host threads by
+  // window: windows are independent
+  std::vector<std::vector<int>> wloc(ws->nwin);
+  auto deal = [&](int w) {
     const int wlo = wp[w], whi = wp[w + 1];
-    off[w] = (int)local.size();
-    if (whi <= wlo) continue;
+    std::vector<int> &loc = wloc[w];
+    if (whi <= wlo) return;
     long links = 0;
     for (int q = wlo + 1; q < whi; q++) links += link[q] != 0;
     const bool chained = 2 * links >= (long)(whi - wlo - 1) && whi - wlo > 1;
@@ -936,8 +965,7 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
       if (link[q + 1]) run[q - wlo] = run[q + 1 - wlo] + 1;
     int start = wlo;
     bool pure = true;  // every link inside the current batch is pure
-    local.push_back(0);
-    first.push_back(wlo);
+    loc.push_back(0);
     for (int q = wlo; q < whi; q++) {
       const int len = q - start;
       // a run of pure links grows past a wavefront into a wide batch of up
@@ -952,12 +980,30 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
       if (q > wlo && ((len >= kWave && !wide_ok) || brk)) {
         start = q;
         pure = true;
-        local.push_back(q - wlo);
-        first.push_back(q);
+        loc.push_back(q - wlo);
       } else if (q > start) {
         pure = pure && link[q] == 2;
       }
-      owner[q] = (int)first.size() - 1;
+    }
+  };
+  {
+    const int nthr = (int)std::max<unsigned>(
+        1, std::min<unsigned>({16u, std::thread::hardware_concurrency(),
+                               (unsigned)(n / 65536 + 1)}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; t++)
+      pool.emplace_back([&, t] {
+        for (int w = t; w < ws->nwin; w += nthr) deal(w);
+      });
+    for (int w = 0; w < ws->nwin; w += nthr) deal(w);
+    for (std::thread &th : pool) th.join();
+  }
+  std::vector<int> local, first, off(ws->nwin + 1, 0);
+  for (int w = 0; w < ws->nwin; w++) {
+    off[w] = (int)local.size();
+    for (int v : wloc[w]) {
+      local.push_back(v);
+      first.push_back(wp[w] + v);
     }
   }
   off[ws->nwin] = (int)local.size();
@@ -973,7 +1019,7 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   ST_TRY(ws->bstart.ensure(sizeof(int) * nb, false));
   ST_TRY(ws->boff.ensure(sizeof(int) * off.size(), false));
   ST_TRY(bfirst->ensure(sizeof(int) * (nb + 1), false));
-  ST_TRY(bid->ensure(sizeof(int) * owner.size(), false));
+  ST_TRY(bid->ensure(sizeof(int) * (size_t)std::max(n, 1), false));
   first.push_back(n);  // the end of the last batch
   if (!local.empty()) {
     HIP_TRY(hipMemcpyAsync(ws->bstart.p, local.data(), sizeof(int) * local.size(),
@@ -983,8 +1029,12 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   }
   HIP_TRY(hipMemcpyAsync(ws->boff.p, off.data(), sizeof(int) * off.size(),
                          hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(bid->p, owner.data(), sizeof(int) * owner.size(),
-                         hipMemcpyHostToDevice, s));
+  // each position's batch, from the batch starts
+  if (n > 0 && !local.empty()) {
+    k_batch_ids<<<grid_for(local.size(), 256), 256, 0, s>>>(
+        bfirst->as<int>(), (int)local.size(), bid->as<int>());
+    HIP_TRY(hipGetLastError());
+  }
   // processing order: each window's batches by (level in the batch DAG of
   // the whole factor, index).  Every batch's in-window predecessors have
   // lower levels, so the wavefront owning the lowest unfinished batch of
@@ -996,6 +1046,7 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   // the previous window's head) ahead of this plane's (inputs from its
   // tail).  The levels come from the device sweep over the batch DAG
   // (k_unit_levels with the batches as units).
+  pt.mark("batches: host deal + upload");
   std::vector<int> lev(nb, 0);
   if (n > 0 && !local.empty()) {
     DevBuf dlev;
@@ -1009,6 +1060,7 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
     dlev.release();
     if (st != SPTRSV_OK) return st;
   }
+  pt.mark("batches: levels (incl. CSR)");
   std::vector<int> order(nb, 0);
   for (int w = 0; w < ws->nwin; w++) {
     const int b0 = off[w], b1 = off[w + 1];
@@ -1035,6 +1087,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
                      hipStream_t s) {
   if (ws->nslots >= 0) return SPTRSV_OK;
   const int n = p->n;
+  PhaseTimer pt(s);
   ST_TRY(ws->rptr.ensure(sizeof(int) * ((size_t)n + 1), false));
   ST_TRY(ws->etgt.ensure(sizeof(int) * (size_t)std::max(p->nnz, 1), false));
   if (n > 0) {
@@ -1063,10 +1116,12 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
   }
   // batches and in-batch predecessors delivered by register (must follow
   // k_lane_targets, whose etgt values it overwrites for the selected entries)
+  pt.mark("slot targets");
   ST_TRY(ws->chain.ensure(sizeof(int2) * (size_t)std::max(n, 1), false));
   DevBuf bfirst, bid;
   std::vector<int> boff_h, bord_h;
   ST_TRY(build_batches(p, ws, &bfirst, &bid, &boff_h, &bord_h, s));
+  pt.t = std::chrono::steady_clock::now();
   if (n > 0) {
     DevBuf key;
     ST_TRY(key.ensure(sizeof(unsigned long long) * (size_t)n, true));
@@ -1090,6 +1145,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     key.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
   }
+  pt.mark("chains");
   // receive lists: the columns with remote inputs, grouped by window and
   // within a window in the batches' processing order, so receivers (which
   // finish their entries in list order) take a row's arrivals when its
@@ -1163,6 +1219,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     bid.release();
     if (st != SPTRSV_OK) return st;
   }
+  pt.mark("receive lists");
   ws->nslots = nslots;
   return SPTRSV_OK;
 }
@@ -1322,7 +1379,12 @@ int prepare_exec_impl(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     L->grid = (int)std::max(1LL, std::min<long long>(grid, blocks_needed));
     return SPTRSV_OK;
   }
-  ST_TRY(get_windows(p, c, &L->ws, s));
+  {
+    PhaseTimer pt(s);
+    const size_t nws = p->windows.size();
+    ST_TRY(get_windows(p, c, &L->ws, s));
+    if (p->windows.size() != nws) pt.mark("windows");
+  }
   if (c.engine == SPTRSV_ENGINE_LANE) {
     ST_TRY(get_lane_targets(p, c, L->ws, s));
     L->R = c.rhs_per_lane;

@@ -887,6 +887,13 @@ __global__ void k_remote_flags(const int2 *__restrict__ need, int n,
     count[p] = (need[backward ? n - 1 - p : p].y + kRecvChunk - 1) / kRecvChunk;
 }
 
+// bid[p] = the batch holding solve position p (bfirst: nb + 1 batch starts).
+__global__ void k_batch_ids(const int *__restrict__ bfirst, int nb, int *bid) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+       b += gridDim.x * blockDim.x)
+    for (int p = bfirst[b]; p < bfirst[b + 1]; p++) bid[p] = b;
+}
+
 // bpos[b] = pos[first column of batch b] for b < nb; bpos[nb] = pos[n].
 __global__ void k_batch_entry_pos(const int *__restrict__ bfirst,
                                   const int *__restrict__ pos, int nb, int n,

@@ -15,11 +15,11 @@
 // level is 0 when no row in it reads a column of another unit, else one more
 // than the largest level among the units it reads.
 //
-// Units are claimed through a ticket in solve order, kLevelClaim rounds of
-// 64 / G units per claim (G lanes per unit): a claimed unit reads only lower
-// units, claimed earlier by running wavefronts, and every wavefront finishes
-// its units in claim order, so the lowest unfinished unit always has its
-// inputs and the sweep completes with any grid size.
+// Units are claimed through a ticket in solve order, by wavefronts: a claim
+// is `rounds` rounds of 64 / G units (G lanes per unit).  A claimed unit reads
+// only lower units, claimed earlier by running wavefronts, and a wavefront
+// finishes its units in claim order, so the lowest unfinished unit always
+// has its inputs and the sweep completes with any grid size.
 #pragma once
 
 #include "sptrsv_device.hpp"
@@ -34,13 +34,13 @@ struct UnitLevelArgs {
   const int *__restrict__ uid;     // [n] unit of each solve position, or null
   int *lev;                        // [nunits], -1 until set
   int n, nunits, backward;
+  int rounds;                      // rounds of 64 / G units per claim
   int *ticket;                     // [2]: claims, finished workgroups
   int *err;
   long long wait_cap;
 };
 
-constexpr int kLevelClaim = 4;  // rounds of 64 / G units per ticket claim
-constexpr int kLevelU = 4;      // entries per lane per poll round
+constexpr int kLevelU = 4;  // entries per lane per poll round
 
 template <int G>
 __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
@@ -48,6 +48,7 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
   const int lane = __lane_id();
   const int gl = lane & (G - 1);
   const int grp = lane / G;
+  const int chunk = a.rounds * kPer;  // units per claim
   const unsigned long long gmask =
       (G == kWave ? ~0ull : ((1ull << (G % kWave)) - 1)) << (lane & ~(G - 1));
   const int n = a.n;
@@ -55,11 +56,13 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
   int level = 0;
   bool stopped = false;
   for (;;) {
-    int base = 0;
-    if (lane == 0) base = stopped ? a.nunits : atomicAdd(a.ticket, kLevelClaim * kPer);
-    base = __shfl(base, 0);
-    if (base >= a.nunits || stopped) break;
-    for (int rd = 0; rd < kLevelClaim && !stopped; rd++) {
+    // no claim ahead: units in solve order form chains, and a unit held
+    // while its wavefront works on earlier ones would stall its dependents
+    int claim = 0;
+    if (lane == 0) claim = stopped ? a.nunits : atomicAdd(a.ticket, chunk);
+    const int base = __shfl(claim, 0);
+    if (base >= a.nunits) break;
+    for (int rd = 0; rd < a.rounds && !stopped; rd++) {
       const int u = base + rd * kPer + grp;
       const bool act = u < a.nunits;
       int f = 0, j = 0, hi = 0;
@@ -72,12 +75,12 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
       }
       bool fin = !act || j >= hi;
       bool pub = !act;
-      bool chunk = false;
+      bool chunk_in = false;
       unsigned pend = 0;
       int dep[kLevelU];
       int m = 0;  // largest level read + 1
       while (__ballot(!pub) != 0) {
-        if (!fin && !chunk) {
+        if (!fin && !chunk_in) {
 #pragma unroll
           for (int q = 0; q < kLevelU; q++) {
             const int jj = j + q * G;
@@ -91,7 +94,7 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
               }
             }
           }
-          chunk = true;
+          chunk_in = true;
         }
         unsigned got = 0;
 #pragma unroll
@@ -104,8 +107,8 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
             }
           }
         pend &= ~got;
-        if (chunk && pend == 0) {
-          chunk = false;
+        if (chunk_in && pend == 0) {
+          chunk_in = false;
           j += G * kLevelU;
           fin = j >= hi;
         }
