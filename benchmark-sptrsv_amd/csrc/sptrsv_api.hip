@@ -541,7 +541,8 @@ int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
     if (c.lanes_per_column != 1 || c.rhs_lanes != 1 ||
         !(R == 1 || R == 2 || R == 4 || R == 8))
       return SPTRSV_ERR_INVALID_ARG;
-    if (c.block < 128 || c.block > 1024 || c.block % 64 != 0 || c.window < 1)
+    if (c.block < 128 || c.block > lane_max_block(R) || c.block % 64 != 0 ||
+        c.window < 1)
       return SPTRSV_ERR_INVALID_ARG;
     if (c.partition != SPTRSV_COL_PARTITION &&
         c.partition != SPTRSV_NNZ_PARTITION)
@@ -2289,7 +2290,7 @@ int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg)
     if (c.lanes_per_column == 0) c.lanes_per_column = 1;
     if (c.rhs_lanes == 0) c.rhs_lanes = 1;
     if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
-    if (c.block == 0) c.block = 1024;
+    if (c.block == 0) c.block = lane_max_block(c.rhs_per_lane);
     if (c.window == 0) c.window = 4096;
   }
   if (c.lanes_per_column == 0) c.lanes_per_column = d.lanes_per_column;

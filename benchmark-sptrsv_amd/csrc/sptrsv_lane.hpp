@@ -242,37 +242,42 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
   const int C = (len + kWave - 1) / kWave;
   const int k0 = base + lane * C;
   const int mine = max(0, min(C, bend - k0));
-  T cf[kWideC];                      // coefficient of the previous column's x
   T d0 = T(1), iv0 = T(1);           // the batch's first column (lane 0)
   T A[R];
 #pragma unroll
   for (int q = 0; q < R; q++) A[q] = T(0);
   T K = T(1), P = T(1);
+  // coefficient of the previous column's x in column k (0 for the batch's
+  // first column); loaded again for the expansion after the publish rather
+  // than kept in kWideC registers per lane
+  auto coef = [&](int k, T *d_out, T *inv_out) {
+    const int p = wlo + k;
+    const int c = BWD ? n - 1 - p : p;
+    const T d = a.val[BWD ? a.colPtr[c + 1] - 1 : a.colPtr[c]];
+    const T inv = T(1) / d;
+    *d_out = d;
+    *inv_out = inv;
+    if (k == base) return T(0);
+    const int cp = BWD ? n - p : p - 1;  // the pure link from the column at p - 1
+    const T lv = a.val[BWD ? a.colPtr[cp + 1] - 2 : a.colPtr[cp] + 1];
+    return -wide_quot(lv, d, inv);
+  };
+  for (int m = 0; m < mine; m++) {
+    T d, inv;
+    const T cm = coef(k0 + m, &d, &inv);
+    if (k0 + m > base) {
 #pragma unroll
-  for (int m = 0; m < kWideC; m++) {
-    cf[m] = T(0);
-    if (m < mine) {
-      const int p = wlo + k0 + m;
-      const int c = BWD ? n - 1 - p : p;
-      const T d = a.val[BWD ? a.colPtr[c + 1] - 1 : a.colPtr[c]];
-      const T inv = T(1) / d;
-      if (k0 + m > base) {  // the pure link from the column at p - 1
-        const int cp = BWD ? n - p : p - 1;
-        const T lv = a.val[BWD ? a.colPtr[cp + 1] - 2 : a.colPtr[cp] + 1];
-        cf[m] = -wide_quot(lv, d, inv);
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-          const T qv = wide_quot(s_val[(k0 + m) * R + q], d, inv);
-          s_val[(k0 + m) * R + q] = qv;
-          A[q] = __builtin_fma(cf[m], A[q], qv);
-        }
-      } else {
-        d0 = d;
-        iv0 = inv;
+      for (int q = 0; q < R; q++) {
+        const T qv = wide_quot(s_val[(k0 + m) * R + q], d, inv);
+        s_val[(k0 + m) * R + q] = qv;
+        A[q] = __builtin_fma(cm, A[q], qv);
       }
-      K = cf[m] * K;
-      if (m > 0) P = cf[m] * P;
+    } else {
+      d0 = d;
+      iv0 = inv;
     }
+    K = cm * K;
+    if (m > 0) P = cm * P;
   }
   // the last column's entries (they leave the batch), first kPF in registers
   const int kl = bend - 1;
@@ -322,15 +327,15 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
     x[q] = __shfl(A[q], lane > 0 ? lane - 1 : 0);
     if (lane == 0) x[q] = T(0);
   }
+  for (int m = 0; m < mine; m++) {
+    T d, inv;
+    const T cm = coef(k0 + m, &d, &inv);
 #pragma unroll
-  for (int m = 0; m < kWideC; m++)
-    if (m < mine) {
-#pragma unroll
-      for (int q = 0; q < R; q++) {
-        x[q] = __builtin_fma(cf[m], x[q], s_val[(k0 + m) * R + q]);
-        s_val[(k0 + m) * R + q] = x[q];
-      }
+    for (int q = 0; q < R; q++) {
+      x[q] = __builtin_fma(cm, x[q], s_val[(k0 + m) * R + q]);
+      s_val[(k0 + m) * R + q] = x[q];
     }
+  }
   return true;
 }
 
@@ -680,8 +685,12 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   }
 }
 
+// Largest lane-engine workgroup for R right-hand sides per lane: 8 values
+// per lane need the register budget of 512-thread workgroups.
+constexpr int lane_max_block(int R) { return R >= 8 ? 512 : 1024; }
+
 template <typename T, bool BWD, int R, bool WIDE>
-__global__ void __launch_bounds__(1024) k_exec_lane(LaneArgs<T> a) {
+__global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
   // maxw window rows plus 64 dummy rows (one per lane) for branch-free
