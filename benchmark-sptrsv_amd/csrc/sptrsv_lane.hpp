@@ -125,6 +125,14 @@ __device__ __forceinline__ void scan_step(T (&a)[R], T &c) {
   c = c * c1;
 }
 
+// A scan step whose coefficient c (this lane's composed coefficient before
+// the step) was computed ahead.
+template <typename T, int R, int CTRL, int ROWMASK>
+__device__ __forceinline__ void scan_step_pc(T (&a)[R], T c) {
+#pragma unroll
+  for (int q = 0; q < R; q++) a[q] = __builtin_fma(c, dpp_t<CTRL, ROWMASK>(T(0), a[q]), a[q]);
+}
+
 // Inclusive scan over the 64 lanes: row_shr 1, 2, 4, 8 inside each row of
 // 16, then row_bcast:15 into rows 1 and 3, then row_bcast:31 into rows 2
 // and 3 (the gfx9 DPP controls 0x111..0x118, 0x142, 0x143).
@@ -456,6 +464,24 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     T xs[R];  // x of the lanes solved in earlier rounds
 #pragma unroll
     for (int q = 0; q < R; q++) xs[q] = T(0);
+    // the coefficient half of the chain scan depends only on the factor: its
+    // six steps are composed while the batch waits, for the first round
+    T pc[6];
+    const unsigned long long act0 = __ballot(active);
+    if (chain_batch) {
+      T c = active ? coef : T(0);
+      pc[0] = c;
+      c = c * dpp_t<0x111, 0xf>(T(1), c);
+      pc[1] = c;
+      c = c * dpp_t<0x112, 0xf>(T(1), c);
+      pc[2] = c;
+      c = c * dpp_t<0x114, 0xf>(T(1), c);
+      pc[3] = c;
+      c = c * dpp_t<0x118, 0xf>(T(1), c);
+      pc[4] = c;
+      c = c * dpp_t<0x142, 0xa>(T(1), c);
+      pc[5] = c;
+    }
     // ready-lane mask of the last round that solved nothing: until it
     // changes no lane can become solvable, so the round is not repeated
     unsigned long long stale = 0;
@@ -485,7 +511,16 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           // scan every cf is 0 and av is x wherever the segment is ready
           const unsigned long long linked = __ballot(cf != T(0));
           const unsigned long long ready = __ballot(ok);
-          chain_scan<T, R>(av, cf);
+          if (__ballot(active) == act0) {
+            scan_step_pc<T, R, 0x111, 0xf>(av, pc[0]);
+            scan_step_pc<T, R, 0x112, 0xf>(av, pc[1]);
+            scan_step_pc<T, R, 0x114, 0xf>(av, pc[2]);
+            scan_step_pc<T, R, 0x118, 0xf>(av, pc[3]);
+            scan_step_pc<T, R, 0x142, 0xa>(av, pc[4]);
+            scan_step_pc<T, R, 0x143, 0xc>(av, pc[5]);
+          } else {
+            chain_scan<T, R>(av, cf);
+          }
           ok = chain_ready(linked, ready, lane);
         } else {
           int pp = active ? pl : lane;
@@ -541,7 +576,8 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           fence_rel_wg();
           if (solved) {
 #pragma unroll
-            for (int e = 0; e < kPF; e++) add_wg(&s_cnt[ladr[e]], linc[e]);
+            for (int e = 0; e < kPF; e++)
+              add_wg(&s_cnt[ladr[e]], linc[e]);
           }
           // entries past the registers, kLong at a time: loads issued
           // together, one wait per group; each group's LDS values are
