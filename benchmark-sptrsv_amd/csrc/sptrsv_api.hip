@@ -418,6 +418,7 @@ struct sptrsv_plan_s {
   DevBuf val32, csrPtr, csrIdx, csrVal, xtmp, r32, d32, csrVal32;
   DevBuf farcnt;  // unsigned long long: entries >= kFarDistance off the diagonal
   DevBuf lvlOrder;  // int[n]: solve positions sorted by DAG level
+  DevBuf lvlRows;   // int4[n]: row engine's row table in level order
   bool csr_ready = false;
   long long wait_cap_us = kDefaultWaitCapUs;
   long long clock_khz = 100000;  // wall_clock64 rate (queried at creation)
@@ -1359,6 +1360,13 @@ int prepare_exec_impl(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     const T *rv = nullptr;
     ST_TRY(row_values<T>(p, s, &rv));
     ST_TRY(plan_levels(p, s));  // items are claimed in level order
+    if (!p->lvlRows.p && p->n > 0) {
+      ST_TRY(p->lvlRows.ensure(sizeof(int4) * (size_t)p->n, false));
+      k_level_rows<<<grid_for(p->n, 256), 256, 0, s>>>(
+          p->lvlOrder.as<int>(), p->csrPtr.as<int>(), p->n,
+          p->subst == SPTRSV_SUBSTITUTION_BACKWARD, p->lvlRows.as<int4>());
+      HIP_TRY(hipGetLastError());
+    }
     L->rowval = rv;
     L->R = c.rhs_lanes * c.rhs_per_lane;  // tile width
     L->ntiles = (rhs + L->R - 1) / L->R;
@@ -1489,7 +1497,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     ra.val = reinterpret_cast<const T *>(L.rowval);
     ra.b = b;
     ra.x = x;
-    ra.order = p->lvlOrder.as<int>();
+    ra.rows = p->lvlRows.as<int4>();
     ra.n = p->n;
     ra.rhs = rhs;
     ra.ntiles = L.ntiles;
@@ -2269,7 +2277,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
   DevBuf *bufs[] = {&p->inDegree, &p->sync,    &p->gsum,    &p->gcnt,
                     &p->slots64,  &p->slots32, &p->val32,   &p->csrPtr,
                     &p->csrIdx,   &p->csrVal,  &p->xtmp,    &p->r32,
-                    &p->d32,      &p->csrVal32, &p->farcnt, &p->lvlOrder};
+                    &p->d32,      &p->csrVal32, &p->farcnt, &p->lvlOrder, &p->lvlRows};
   for (DevBuf *b : bufs) b->release();
   delete p;
   return SPTRSV_OK;
@@ -2411,7 +2419,7 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
   const DevBuf *bufs[] = {&p->inDegree, &p->sync,    &p->gsum,  &p->gcnt,
                           &p->slots64,  &p->slots32, &p->val32, &p->csrPtr,
                           &p->csrIdx,   &p->csrVal,  &p->xtmp,  &p->r32,
-                          &p->d32,      &p->csrVal32, &p->farcnt, &p->lvlOrder};
+                          &p->d32,      &p->csrVal32, &p->farcnt, &p->lvlOrder, &p->lvlRows};
   for (const DevBuf *b : bufs) bytes += (long long)b->bytes;
   for (auto &kv : p->windows)
     bytes += (long long)(kv.second->winPtr.bytes + kv.second->need.bytes +

@@ -165,4 +165,17 @@ __global__ void k_level_scatter(const int *__restrict__ lev, int len, int *off,
     order[atomicAdd(&off[lev[i]], 1)] = i;
 }
 
+// Row table of the level order: {row, its first entry, the end of its
+// entries} for each place, so a claimed item needs one load to find its row.
+__global__ void k_level_rows(const int *__restrict__ order,
+                             const int *__restrict__ rowPtr, int n, int backward,
+                             int4 *rows) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += gridDim.x * blockDim.x) {
+    const int p = order[k];
+    const int r = backward ? n - 1 - p : p;
+    rows[k] = make_int4(r, rowPtr[r], rowPtr[r + 1], 0);
+  }
+}
+
 }  // namespace sptrsv

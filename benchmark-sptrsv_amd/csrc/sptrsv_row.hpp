@@ -40,7 +40,8 @@ namespace sptrsv {
 template <typename T>
 struct RowArgs {
   const int *__restrict__ rowPtr;  // [n+1] CSR of the triangle
-  const int *__restrict__ order;   // [n] solve positions in level order
+  const int4 *__restrict__ rows;   // [n] in level order: {row, first entry,
+                                   // end of its entries, -}
   const int *__restrict__ colIdx;  // [nnz] columns ascending in each row
   const T *__restrict__ val;       // [nnz]
   const T *__restrict__ b;         // [n][rhs]
@@ -130,9 +131,9 @@ __global__ void __launch_bounds__(1024) k_exec_row(RowArgs<T> a) {
       if (act) {
         const int idx = (int)(t / a.ntiles);
         tile = (int)(t - (long long)idx * a.ntiles);
-        const int p = a.order[idx];
-        r = BWD ? a.n - 1 - p : p;
-        const int s0 = a.rowPtr[r], s1 = a.rowPtr[r + 1];
+        const int4 ri = a.rows[idx];
+        r = ri.x;
+        const int s0 = ri.y, s1 = ri.z;
         d = a.val[BWD ? s0 : s1 - 1];
 #pragma unroll
         for (int q = 0; q < RPL; q++) {
