@@ -251,6 +251,12 @@ def main():
     if args.config:
         tuned = json.loads(args.config)
     elif not args.no_tune:
+        # the tuner's result reaches the timed plan through the tune cache
+        # (a cache hit stores it for the solve type of `mode`)
+        if not os.environ.get("SPTRSV_TUNE_CACHE"):
+            import tempfile
+            os.environ["SPTRSV_TUNE_CACHE"] = os.path.join(
+                tempfile.mkdtemp(prefix="sptrsv_bench_"), "tune.cache")
         tplan = new_plan()
         tuned, tms = tplan.autotune(rhs=rhs, mode=mode)
         log(f"autotuned config {tuned} ({tms:.3f} ms)")
@@ -258,8 +264,10 @@ def main():
     # the timed plan: created afresh, so its analysis times are those of the
     # configuration that is timed (plan creation + per-configuration analysis)
     plan = new_plan()
-    if tuned:
+    if args.config:
         plan.set_config(rhs, **tuned)
+    elif tuned:
+        plan.autotune(rhs=rhs, mode=mode)  # cache hit: no timing runs
     plan.prepare(rhs, mode, stream=stream)
     graph = None
     if args.graph:

@@ -848,6 +848,7 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
 int plan_levels(sptrsv_plan_s *p, hipStream_t s) {
   if (p->nlevel > 0 || p->n == 0) return SPTRSV_OK;
   const int n = p->n;
+  const auto t0 = std::chrono::steady_clock::now();
   DevBuf lev, cnt;
   ST_TRY(lev.ensure(sizeof(int) * (size_t)n, false));
   int st = cnt.ensure(sizeof(int) * ((size_t)n + 1), false);
@@ -908,6 +909,8 @@ int plan_levels(sptrsv_plan_s *p, hipStream_t s) {
   p->lvl_max = mxw;
   p->lvl_avg = n / nl;
   p->nlevel = nl;
+  p->prepare_ms += std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t0).count();
   return SPTRSV_OK;
 }
 

@@ -310,9 +310,11 @@ typedef struct {
                                 sptrsv_plan_autotune computed it)       */
   long long scratch_bytes;   /* device scratch held by the plan        */
   double analysis_ms;        /* device analysis time at creation       */
-  double prepare_ms;         /* per-configuration analyses so far (the
-                                first solve of each configuration, or
-                                sptrsv_plan_prepare), host wall clock  */
+  double prepare_ms;         /* analysis after creation so far: the
+                                per-configuration analyses (first solve
+                                of each configuration, or
+                                sptrsv_plan_prepare) and the level
+                                analysis, host wall clock               */
 } sptrsv_plan_info_t;
 int sptrsv_plan_info(sptrsv_plan_t plan, sptrsv_plan_info_t *info);
 
