@@ -51,6 +51,9 @@ def make_workload(name):
     if name == "bidiag":
         f = mx.bidiagonal(1_000_000)
         return f, mx.x_uniform(f.n, seed=0), 1_000_000
+    if name == "chain_far":
+        f = mx.chain_far(1_000_000)
+        return f, mx.x_uniform(f.n, seed=0), 1_000_000
     if name == "batch":
         f = mx.laplacian2d_batch(512, seed=0)
         return f, mx.x_uniform(f.n, seed=0), None
@@ -74,6 +77,9 @@ WORKLOAD_DESC = {
     "powerlaw": "random power-law lower triangular n=1,000,000, Zipf(2) cap 1000, window 50,000",
     "bidiag": "bidiagonal chain n=1,000,000, diag 2, sub -1",
     "batch": "512 uneven 2D 5-point Laplacian lower factors (sides log-U[32,256]) in one launch",
+    "chain_far": "chain n=1,000,000 (diag 2, sub -1) with a second input 9000 columns back "
+                 "(-0.5): no pure links, so it is passed one 64-column batch at a time; "
+                 "time / 15,625 = the cost of one batch-to-batch step",
     "stencil27": "lower triangle of 3D 27-point stencil 100x100x60 (n=600,000), diag 26, off -1",
     "lap3d_share": "one GPU's share of the 128^3 3D Laplacian factor under an 8-way "
                    "contiguous column split (diagonal block of 16 planes, n=262,144; "

@@ -822,19 +822,25 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
   a.err = p->sync.as<int>() + kErrWord;
   // analysis is not subject to the solve's spin cap
   a.wait_cap = kDefaultWaitCapUs * p->clock_khz / 1000;
-  // rows (few entries each): 16 lanes per row, 4 rounds of 4 rows per
-  // wavefront claim; batches (up to hundreds of entries): a wavefront per
-  // batch, one batch per claim
-  const int G = d_ufirst ? 64 : 16;
-  a.rounds = d_ufirst ? 1 : 4;
+  // One round per claim: a claim's units are worked on at once, one per
+  // lane group.  Several rounds per claim would chain them: a claim that
+  // straddles the end of a grid line would hold the next line's first rows
+  // behind the current line's last, and the lines would be swept one after
+  // another (a 700^2 grid took 372 ms that way, 512 uneven grids 4 s).
+  // Rows: 4 lanes per row when rows are short (16 rows per claim), 16 when
+  // they are long; batches (up to hundreds of entries): a wavefront each.
+  const int G = d_ufirst ? 64 : (p->avg_offdiag <= 8.0 ? 4 : 16);
+  a.rounds = 1;
   const long long per_claim = (long long)a.rounds * (kWave / G);
   const long long claims = (nunits + per_claim - 1) / per_claim;
   const int grid =
       (int)std::max(1LL, std::min<long long>((claims + 3) / 4, (long long)p->cus * 8));
   if (G == 64)
     k_unit_levels<64><<<grid, 256, 0, s>>>(a);
-  else
+  else if (G == 16)
     k_unit_levels<16><<<grid, 256, 0, s>>>(a);
+  else
+    k_unit_levels<4><<<grid, 256, 0, s>>>(a);
   HIP_TRY(hipGetLastError());
   ST_TRY(sptrsv_plan_check(p, s));
   return SPTRSV_OK;

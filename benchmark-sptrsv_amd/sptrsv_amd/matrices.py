@@ -132,6 +132,22 @@ def bidiagonal(n, diag=2.0, sub=-1.0):
     return Factor(p, ri, vv, name=f"bidiag_{n}")
 
 
+def chain_far(n, far=9000, diag=2.0, sub=-1.0, farval=-0.5):
+    """A bidiagonal chain whose rows also take an entry `far` columns back:
+    L(i, i) = diag, L(i, i-1) = sub, L(i, i-far) = farval (i >= far).  The
+    DAG depth is n (the chain), but no link is a pure first-order recurrence
+    (row i has a second input and column i-1 a second entry), so a solver
+    passes the chain batch by batch: its time over n/64 is the cost of one
+    step from a 64-column batch to the next.  Diagonally dominant."""
+    i = np.arange(n, dtype=np.int64)
+    fi = i[far:]
+    c = np.concatenate([i, i[:-1], fi - far])
+    r = np.concatenate([i, i[:-1] + 1, fi])
+    v = np.concatenate([np.full(n, diag), np.full(n - 1, sub), np.full(fi.size, farval)])
+    p, ri, vv = _from_columns(n, c, r, v)
+    return Factor(p, ri, vv, name=f"chain_far_{n}_{far}")
+
+
 def power_law_lower(n, s=2.0, cap=1000, window=50000, seed=0):
     """Random power-law lower factor.
 
