@@ -168,3 +168,19 @@ def test_graph_keeps_scratch_in_place():
     plan.solve(db8, dx8, rhs=8)
     assert plan.check() == 0
     assert_close(dx8.to_host(), oracle_x(f, B8, rhs=8), 1e-12)
+
+
+def test_device_levels_straddling_claims_full_size_batch():
+    # grid lines of 700 and random widths straddle the sweep's claims of 16
+    # rows; the sweep must still pipeline the lines (4 s when it did not)
+    import time
+    for f, closed in [(mx.laplacian2d_lower(700, 40), 700 + 40 - 1),
+                      (mx.laplacian2d_batch(512, seed=0), None)]:
+        plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val, sys_ptr=f.sys_ptr)
+        t = time.perf_counter()
+        got = plan.levels()
+        dt = time.perf_counter() - t
+        assert got == tuple(orc.findlevel(f.colPtr, f.rowIdx))
+        if closed is not None:
+            assert got[0] == closed
+        assert dt < 2.0, f"level sweep took {dt:.2f} s"

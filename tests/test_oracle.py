@@ -174,6 +174,20 @@ def test_stencil27_generator_matches_brute_force():
     assert np.array_equal(A, np.tril(M))
 
 
+def test_chain_far_generator_matches_brute_force():
+    n, far = 40, 7
+    f = mx.chain_far(n, far=far)
+    A = sp.csc_matrix((f.val, f.rowIdx, f.colPtr), shape=(n, n)).toarray()
+    M = 2.0 * np.eye(n) - np.eye(n, k=-1)
+    M[np.arange(far, n), np.arange(0, n - far)] = -0.5
+    assert np.array_equal(A, M)
+    # every row is a chain step (depth n) and the chain is diagonally dominant
+    assert orc.findlevel(f.colPtr, f.rowIdx)[0] == n
+    x = mx.x_uniform(n, seed=2)
+    np.testing.assert_allclose(orc.solve(f.colPtr, f.rowIdx, f.val, f.spmv(x)), x,
+                               rtol=0, atol=1e-14)
+
+
 def test_rel_err_formula():
     x = np.array([1.0, 2.0, -3.0])
     xr = np.array([1.0, 2.5, -2.0])
