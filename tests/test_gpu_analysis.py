@@ -137,7 +137,9 @@ def test_tune_cache_key_separates_depths(tmp_path):
 
 
 def test_graph_keeps_scratch_in_place():
-    f = mx.laplacian3d_lower(24, 20, 16)
+    # several default windows (5 planes of 1600 columns each), so the solve
+    # uses cross-window slots, 8 times as many with 8 right-hand sides
+    f = mx.laplacian3d_lower(40, 40, 16)
     B1 = f.spmv(mx.x_uniform(f.n, seed=3))
     plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
     db = S.DeviceArray.from_host(B1)
