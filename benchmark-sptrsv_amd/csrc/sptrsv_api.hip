@@ -429,6 +429,7 @@ struct sptrsv_plan_s {
   double avg_offdiag = 0.0;
   double far_frac = 0.0;  // off-diagonal entries >= kFarDistance from the diagonal
   int period = 0;         // most common column reach (grid plane/row), or 0
+  bool chain = false;     // most columns reach only the next row
   // captured graphs hold raw pointers to the scratch buffers: while any
   // exists, scratch is never reallocated (a larger request is refused)
   int live_graphs = 0;
@@ -466,6 +467,25 @@ int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
   if (D <= wmax) return D;  // one period per window
   const int k = (D + wmax - 1) / wmax;  // else each period split evenly
   return (D + k - 1) / k;
+}
+
+// How a lane-engine configuration delivers entries from earlier windows
+// when it leaves that to the library: directly where one right-hand side
+// per lane meets windows that hold whole grid periods (a plane, a row, a
+// chain), through receivers otherwise (windows that split a plane leave
+// rows with two remote entries, and the receivers busy; batches of
+// systems).  Other engines carry 0.
+void fill_remote(const sptrsv_plan_s *p, sptrsv_config_t *c) {
+  if (c->engine != SPTRSV_ENGINE_LANE) {
+    c->remote = 0;
+    return;
+  }
+  if (c->remote != 0) return;
+  const bool whole = p->nsys == 1 &&
+                     ((p->period > 0 && p->period <= c->window) ||
+                      (p->period == 0 && p->chain));
+  c->remote = c->rhs_per_lane == 1 && whole ? SPTRSV_REMOTE_DIRECT
+                                           : SPTRSV_REMOTE_RECEIVER;
 }
 
 // Row-engine lane layouts compiled: (lanes per row, lanes over right-hand
@@ -520,6 +540,7 @@ sptrsv_config_t default_config(const sptrsv_plan_s *p, int rhs) {
   c.window = lane_window(p, c.rhs_per_lane, p->vt == SPTRSV_F64 ? 8 : 4);
   c.partition = SPTRSV_COL_PARTITION;
   c.grid = 0;
+  fill_remote(p, &c);
   return c;
 }
 
@@ -535,6 +556,7 @@ int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
     if (c.window < 1 || c.window > 64) return SPTRSV_ERR_INVALID_ARG;
     if (c.block < 64 || c.block > 1024 || c.block % 64 != 0)
       return SPTRSV_ERR_INVALID_ARG;
+    if (c.remote != 0) return SPTRSV_ERR_INVALID_ARG;
     return SPTRSV_OK;
   }
   if (c.engine == SPTRSV_ENGINE_LANE) {
@@ -551,8 +573,12 @@ int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
       return SPTRSV_ERR_INVALID_ARG;
     if (lane_lds_bytes(c.window, R, tsize) > p->lds_max)
       return SPTRSV_ERR_INVALID_ARG;
+    if (c.remote != SPTRSV_REMOTE_RECEIVER &&
+        !(c.remote == SPTRSV_REMOTE_DIRECT && R == 1))
+      return SPTRSV_ERR_INVALID_ARG;
     return SPTRSV_OK;
   }
+  if (c.remote != 0) return SPTRSV_ERR_INVALID_ARG;
   if (c.engine != SPTRSV_ENGINE_GROUP) return SPTRSV_ERR_INVALID_ARG;
   if (!is_pow2(c.lanes_per_column) || c.lanes_per_column > 64)
     return SPTRSV_ERR_INVALID_ARG;
@@ -621,7 +647,10 @@ void equal_nnz_partition_host(int nparts, int ncols, int nnz, const int *ptr,
 
 int get_windows(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet **out,
                 hipStream_t s) {
-  const long long key = (long long)c.window * 2 + c.partition;
+  // the receive lists of a lane-engine window set depend on how rows with
+  // one remote entry get it
+  const long long key = ((long long)c.window * 2 + c.partition) * 2 +
+                        (c.remote == SPTRSV_REMOTE_DIRECT ? 1 : 0);
   auto it = p->windows.find(key);
   if (it != p->windows.end()) {
     *out = it->second;
@@ -1169,7 +1198,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     if (n > 0) {
       k_remote_flags<<<grid_for(n, 256), 256, 0, s>>>(
           ws->need.as<int2>(), n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
-          pos.as<int>());
+          c.remote == SPTRSV_REMOTE_DIRECT, pos.as<int>());
       HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemsetAsync(pos.as<int>() + n, 0, sizeof(int), s));
@@ -1219,7 +1248,7 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
           bid.as<int>(), shift.as<int>(),
           ws->winPtr.as<int>(), ws->nwin, n,
           p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
-          ws->uniform_w,
+          ws->uniform_w, c.remote == SPTRSV_REMOTE_DIRECT,
           ws->rlist.as<int4>(), ws->roff.as<int>());
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         st = SPTRSV_ERR_HIP;
@@ -1246,30 +1275,29 @@ DevBuf &slot_buf<float>(sptrsv_plan_s *p) { return p->slots32; }
 template <typename T>
 using LaneKernel = void (*)(LaneArgs<T>);
 
-// Lane-engine kernel for R right-hand sides per lane (1, 2, 4 or 8).
-template <typename T>
-LaneKernel<T> pick_lane_w(bool bwd, int R) {
+// Lane-engine kernel for R right-hand sides per lane (1, 2, 4 or 8), with
+// or without the wide-batch path (compiled in only where the window set has
+// a wide batch: it costs the others a few registers); rows with one remote
+// entry take it directly only with one right-hand side per lane.
+template <typename T, bool BWD, bool WIDE>
+LaneKernel<T> pick_lane_r(int R, bool direct) {
   switch (R) {
-    case 1: return bwd ? k_exec_lane<T, true, 1, true> : k_exec_lane<T, false, 1, true>;
-    case 2: return bwd ? k_exec_lane<T, true, 2, true> : k_exec_lane<T, false, 2, true>;
-    case 4: return bwd ? k_exec_lane<T, true, 4, true> : k_exec_lane<T, false, 4, true>;
-    case 8: return bwd ? k_exec_lane<T, true, 8, true> : k_exec_lane<T, false, 8, true>;
+    case 1: return direct ? k_exec_lane<T, BWD, 1, WIDE, true>
+                          : k_exec_lane<T, BWD, 1, WIDE, false>;
+    case 2: return direct ? nullptr : k_exec_lane<T, BWD, 2, WIDE, false>;
+    case 4: return direct ? nullptr : k_exec_lane<T, BWD, 4, WIDE, false>;
+    case 8: return direct ? nullptr : k_exec_lane<T, BWD, 8, WIDE, false>;
     default: return nullptr;
   }
 }
 
-// The instantiation with wide batches only where the window set has one
-// (the extra path costs the others a few registers).
 template <typename T>
-LaneKernel<T> pick_lane(bool bwd, int R, bool wide) {
-  if (wide) return pick_lane_w<T>(bwd, R);
-  switch (R) {
-    case 1: return bwd ? k_exec_lane<T, true, 1, false> : k_exec_lane<T, false, 1, false>;
-    case 2: return bwd ? k_exec_lane<T, true, 2, false> : k_exec_lane<T, false, 2, false>;
-    case 4: return bwd ? k_exec_lane<T, true, 4, false> : k_exec_lane<T, false, 4, false>;
-    case 8: return bwd ? k_exec_lane<T, true, 8, false> : k_exec_lane<T, false, 8, false>;
-    default: return nullptr;
-  }
+LaneKernel<T> pick_lane(bool bwd, int R, bool wide, bool direct) {
+  if (bwd)
+    return wide ? pick_lane_r<T, true, true>(R, direct)
+                : pick_lane_r<T, true, false>(R, direct);
+  return wide ? pick_lane_r<T, false, true>(R, direct)
+              : pick_lane_r<T, false, false>(R, direct);
 }
 
 // Receiver wavefronts of a lane-engine workgroup: a quarter of them, at least
@@ -1424,7 +1452,7 @@ int prepare_exec_impl(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     }
     LaneKernel<T> kern =
         pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L->R,
-                     L->ws->has_wide);
+                     L->ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     if (L->lds > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)kern,
@@ -1552,7 +1580,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
     LaneKernel<T> kern =
         pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L.R,
-                     L.ws->has_wide);
+                     L.ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     kern<<<L.grid, c.block, L.lds, s>>>(la);
     HIP_TRY(hipGetLastError());
@@ -1709,10 +1737,11 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
       dr.release();
       if (st == SPTRSV_OK) {
         std::sort(reach.begin(), reach.end());
-        int best = 0, best_n = 0;
+        int best = 0, best_n = 0, n1 = 0;
         for (int i = 0; i < nsamp;) {
           int j = i;
           while (j < nsamp && reach[j] == reach[i]) j++;
+          if (reach[i] == 1) n1 = j - i;
           if (reach[i] > 1 && j - i > best_n) {
             best = reach[i];
             best_n = j - i;
@@ -1721,6 +1750,8 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
         }
         // a period when it is the reach of at least 40% of the columns
         p->period = 5 * best_n >= 2 * nsamp ? best : 0;
+        // a chain when at least 40% of the columns reach only the next row
+        p->chain = 5 * n1 >= 2 * nsamp;
       }
     }
     // entries farther from the diagonal than the default lane window
@@ -1798,9 +1829,9 @@ bool tune_cache_lookup(const std::string &key, sptrsv_config_t *c) {
     if (key != line) continue;
     sptrsv_config_t t;
     memset(&t, 0, sizeof(t));
-    if (sscanf(bar + 1, "%d %d %d %d %d %d %d %d", &t.lanes_per_column,
+    if (sscanf(bar + 1, "%d %d %d %d %d %d %d %d %d", &t.lanes_per_column,
                &t.rhs_lanes, &t.rhs_per_lane, &t.block, &t.window,
-               &t.partition, &t.grid, &t.engine) == 8) {
+               &t.partition, &t.grid, &t.engine, &t.remote) == 9) {
       *c = t;
       found = true;  // last entry wins
     }
@@ -1814,9 +1845,9 @@ void tune_cache_store(const std::string &key, const sptrsv_config_t &c) {
   if (!path || !*path) return;
   FILE *f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s|%d %d %d %d %d %d %d %d\n", key.c_str(), c.lanes_per_column,
+  fprintf(f, "%s|%d %d %d %d %d %d %d %d %d\n", key.c_str(), c.lanes_per_column,
           c.rhs_lanes, c.rhs_per_lane, c.block, c.window, c.partition, c.grid,
-          c.engine);
+          c.engine, c.remote);
   fclose(f);
 }
 
@@ -1824,7 +1855,7 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
                                              int tsize) {
   std::vector<sptrsv_config_t> out;
   auto add = [&](int G, int RL, int rpl, int block, int window, int part,
-                 int engine = SPTRSV_ENGINE_GROUP) {
+                 int engine = SPTRSV_ENGINE_GROUP, int remote = 0) {
     sptrsv_config_t c;
     memset(&c, 0, sizeof(c));
     c.engine = engine;
@@ -1834,6 +1865,8 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     c.block = block;
     c.window = window;
     c.partition = part;
+    c.remote = remote;
+    fill_remote(p, &c);
     if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
   };
   for (int R : {1, 2, 4, 8}) {
@@ -1842,15 +1875,22 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     const int aligned = lane_window(p, R, tsize);
     for (int block : {1024, 512, 256})
       add(1, 1, R, block, aligned, 0, SPTRSV_ENGINE_LANE);
+    if (R == 1)  // both ways of taking entries from earlier windows
+      for (int remote : {SPTRSV_REMOTE_RECEIVER, SPTRSV_REMOTE_DIRECT})
+        add(1, 1, R, 1024, aligned, 0, SPTRSV_ENGINE_LANE, remote);
     // windows of balanced nonzero counts (the reference's equal_nnz_partition
     // as one GPU's window ordering) at the same maximum width
     add(1, 1, R, 1024, aligned, SPTRSV_NNZ_PARTITION, SPTRSV_ENGINE_LANE);
     int added = 0;
     for (int window : {8192, 4096, 2048, 1024}) {
       if (lane_lds_bytes(window, R, tsize) > p->lds_max) continue;
-      if (window != aligned)
+      if (window != aligned) {
         for (int block : {1024, 512, 256})
           add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
+        if (R == 1)
+          for (int remote : {SPTRSV_REMOTE_RECEIVER, SPTRSV_REMOTE_DIRECT})
+            add(1, 1, R, 1024, window, 0, SPTRSV_ENGINE_LANE, remote);
+      }
       if (++added == 2) break;
     }
   }
@@ -2322,6 +2362,7 @@ int sptrsv_plan_set_config(sptrsv_plan_t p, int rhs, const sptrsv_config_t *cfg)
   if (c.rhs_per_lane == 0) c.rhs_per_lane = 1;
   if (c.block == 0) c.block = d.block;
   if (c.window == 0) c.window = c.block / c.lanes_per_column;
+  fill_remote(p, &c);
   // the same configuration serves the plan's own type and the FP32 solve of IR
   const int tsize = p->vt == SPTRSV_F64 ? 8 : 4;
   ST_TRY(check_config(p, c, tsize));

@@ -240,7 +240,7 @@ __device__ __forceinline__ void wide_count(const LaneArgs<T> &a, int *s_cnt,
 // the batch's last column (the last of lane `owner`) publishes at once, and
 // then each lane expands its columns from its predecessor lane's last x.
 // Returns false when the stall watch fired.
-template <typename T, bool BWD, int R>
+template <typename T, bool BWD, int R, bool DIRECT>
 __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
                                           int *s_cnt, int base, int bend,
                                           int wlo, int tile, int rv) {
@@ -304,9 +304,25 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
   }
   // the first column's arrivals (its predecessor is in an earlier batch)
   const int pf = wlo + base;
-  const int2 nd = a.need[BWD ? n - 1 - pf : pf];
-  const int need0 = nd.x + nd.y * rv;
+  const int c0 = BWD ? n - 1 - pf : pf;
+  const int2 nd = a.need[c0];
+  const bool direct = DIRECT && nd.y == 1;
+  const int need0 = nd.x + (direct ? 0 : nd.y * rv);
   T *slots = a.slots + (size_t)tile * a.nslots * R;
+  T rem[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) rem[q] = T(0);
+  if (direct && lane == 0) {  // the first column's one remote slot
+    const int sidx = a.rptr[c0];
+    const T empty = slot_empty_value<T>();
+    StallWatch w0;
+    for (int q = 0; q < rv; q++) {
+      T v = empty;
+      while (slot_empty(v = xchg_take(&slots[(size_t)sidx * R + q], empty)))
+        if (w0.step(false, a.err, a.wait_cap)) break;
+      rem[q] = slot_empty(v) ? T(0) : v;
+    }
+  }
   StallWatch watch;
   while (ld_wg(&s_cnt[base]) != need0)
     if (watch.step(false, a.err, a.wait_cap)) return false;
@@ -315,7 +331,7 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < R; q++) {
-      const T a0 = wide_quot(s_val[base * R + q], d0, iv0);
+      const T a0 = wide_quot(s_val[base * R + q] - rem[q], d0, iv0);
       s_val[base * R + q] = a0;
       A[q] = __builtin_fma(P, a0, A[q]);
     }
@@ -364,7 +380,16 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
 // whose chain is not complete wait for the next round; the lowest unsolved
 // lane always completes once its inputs from earlier batches arrive, so the
 // batch always drains.
-template <typename T, bool BWD, int R, bool WIDE>
+//
+// DIRECT: a row with exactly one entry from an earlier window takes that slot
+// in its own lane (an exchange poll issued beside each LDS count read) rather
+// than through a receiver's LDS deposit and counter, one handoff less on a
+// cross-window step.  It pays where windows hold whole grid planes, so that
+// nearly every row is such a row (a 100^3 grid: 0.34 -> 0.23 ms, the 10^6
+// chain's wide batches 0.46 -> 0.36 ms); where half-plane windows leave the
+// receivers busy too, the extra pollers cost more than the handoff saves
+// (128^3: +6%), and the configuration chooses.
+template <typename T, bool BWD, int R, bool WIDE, bool DIRECT>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, const int *s_bst,
                                             const int *s_bord, int nb, int wlo,
@@ -374,8 +399,9 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
   const int n = a.n;
   T *slots = a.slots + (size_t)tile * a.nslots * R;
   bool stopped = false;  // the stall watch fired: leave the window
-  int pf_bi = -1, pf_f = 0, pf_l = 0;
+  int pf_bi = -1, pf_f = 0, pf_l = 0, pf_rp = 0;
   int2 pf_nd = make_int2(0, 0), pf_ch = make_int2(-1, 0);
+  const T empty = slot_empty_value<T>();
   // the window's batches in level order, dealt round-robin: wavefront cw
   // takes positions cw, cw + ncw, ... of that order
   for (int pos = cw; pos < nb && !stopped; pos += ncw) {
@@ -386,7 +412,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     const int bend = bi + 1 < nb ? s_bst[bi + 1] : wlen;
     if constexpr (WIDE) {
       if (bend - base > kWave) {  // a run of pure links: one scan
-        if (!lane_wide<T, BWD, R>(a, s_val, s_cnt, base, bend, wlo, tile, rv))
+        if (!lane_wide<T, BWD, R, DIRECT>(a, s_val, s_cnt, base, bend, wlo, tile, rv))
           stopped = true;
         continue;
       }
@@ -397,20 +423,30 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     T diag = T(1), inv = T(1), coef = T(0);
     int tg[kPF];
     T vv[kPF];
+    // a row with one remote entry takes that slot itself (sidx), its value
+    // kept in rem and subtracted at the solve: no receiver deposit, no LDS
+    // counter between the slot's arrival and the solve
+    bool direct = false;
+    int sidx = 0;
     if (active) {
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
-      int f, l;
+      int f, l, rp;
       int2 nd, ch;
-      if (pf_bi == bi) { f = pf_f; l = pf_l; nd = pf_nd; ch = pf_ch; }
-      else { f = a.colPtr[c]; l = a.colPtr[c + 1]; nd = a.need[c]; ch = a.chain[c]; }
+      if (pf_bi == bi) { f = pf_f; l = pf_l; nd = pf_nd; ch = pf_ch; rp = pf_rp; }
+      else {
+        f = a.colPtr[c]; l = a.colPtr[c + 1]; nd = a.need[c]; ch = a.chain[c];
+        rp = DIRECT ? a.rptr[c] : 0;
+      }
+      direct = DIRECT && nd.y == 1;
+      sidx = rp;
       diag = a.val[BWD ? l - 1 : f];
       inv = T(1) / diag;  // off the critical path
       lo = BWD ? f : f + 1;
       hi = BWD ? l - 1 : l;
       ne = min(hi - lo, kPF);
       // receivers count one arrival per remote entry and right-hand side
-      need_lds = nd.x + nd.y * rv;
+      need_lds = nd.x + (direct ? 0 : nd.y * rv);
       if (ch.x >= 0 && ch.x < lane) {
         pl = ch.x;
         need_lds -= 1;
@@ -441,6 +477,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           pf_l = a.colPtr[cc + 1];
           pf_nd = a.need[cc];
           pf_ch = a.chain[cc];
+          if (DIRECT) pf_rp = a.rptr[cc];
         }
       }
     }
@@ -486,9 +523,32 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     // changes no lane can become solvable, so the round is not repeated
     unsigned long long stale = 0;
     StallWatch watch;
+    T rem[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) rem[q] = T(0);
+    const unsigned rall = direct ? (1u << rv) - 1 : 0u;
+    unsigned rgot = 0;  // bit q: the remote value of right-hand side q taken
     while (__ballot(active) != 0) {
+      // the slot polls are issued before the LDS count is read, so both
+      // round trips overlap
+      T sv[R];
+      const bool poll = DIRECT && active && rgot != rall;
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        sv[q] = empty;
+        if (poll && q < rv && !(rgot & (1u << q)))
+          sv[q] = xchg_take(&slots[(size_t)sidx * R + q], empty);
+      }
       const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
-      const bool ext = active && cnt == need_lds;
+      bool took = false;
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        if (!slot_empty(sv[q])) {
+          rem[q] = sv[q];
+          rgot |= 1u << q;
+          took = true;
+        }
+      const bool ext = active && cnt == need_lds && rgot == rall;
       bool progressed = false;  // some lane solved in this round
       const unsigned long long extm = __ballot(ext);
       if (extm != 0 && extm != stale) {
@@ -499,7 +559,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         T av[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
-          const T r = ext ? s_val[k * R + q] : T(0);
+          const T r = ext ? s_val[k * R + q] - rem[q] : T(0);
           T qv = r * inv;
           qv = __builtin_fma(__builtin_fma(-qv, diag, r), inv, qv);
           av[q] = active ? qv : xs[q];
@@ -623,7 +683,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       // from delaying a solving wavefront, and a sleep would only delay the
       // poll that sees the next arrival.  A wavefront whose lanes have inputs
       // but wait on an unsolved predecessor counts as stalled.
-      if (watch.step(progressed, a.err, a.wait_cap)) {
+      if (watch.step(progressed || __ballot(took) != 0, a.err, a.wait_cap)) {
         active = false;
         stopped = true;
       }
@@ -725,7 +785,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
 // per lane need the register budget of 512-thread workgroups.
 constexpr int lane_max_block(int R) { return R >= 8 ? 512 : 1024; }
 
-template <typename T, bool BWD, int R, bool WIDE>
+template <typename T, bool BWD, int R, bool WIDE, bool DIRECT>
 __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
@@ -769,7 +829,7 @@ __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) 
     if (wave < a.nrecv)
       lane_receive<T, R>(a, s_val, s_cnt, win, tile, rv, wave);
     else
-      lane_compute<T, BWD, R, WIDE>(a, s_val, s_cnt, s_bst, s_bord, nb, wlo,
+      lane_compute<T, BWD, R, WIDE, DIRECT>(a, s_val, s_cnt, s_bst, s_bord, nb, wlo,
                                     wlen, tile, rv, wave - a.nrecv, ncw);
     __syncthreads();
     for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
@@ -924,12 +984,16 @@ __global__ void k_need_remote(const int2 *__restrict__ need, int n, int *out) {
 // instead of one lane group by group.
 constexpr int kRecvChunk = 2 * kTake;
 
-// count[p] = receive-list entries of the column at solve position p.
+// count[p] = receive-list entries of the column at solve position p (none
+// for a row with one remote entry when such rows take their slot directly).
 __global__ void k_remote_flags(const int2 *__restrict__ need, int n,
-                               int backward, int *count) {
+                               int backward, int direct, int *count) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += gridDim.x * blockDim.x)
-    count[p] = (need[backward ? n - 1 - p : p].y + kRecvChunk - 1) / kRecvChunk;
+  {
+    const int m = need[backward ? n - 1 - p : p].y;
+    count[p] = direct && m == 1 ? 0 : (m + kRecvChunk - 1) / kRecvChunk;
+  }
 }
 
 // bid[p] = the batch holding solve position p (bfirst: nb + 1 batch starts).
@@ -952,20 +1016,21 @@ __global__ void k_batch_entry_pos(const int *__restrict__ bfirst,
 // each window's entries are contiguous; roff[w] = pos[winPtr[w]].  Within a
 // window a batch's entries (its columns in solve order) move by shift[batch]
 // to the batch's place in the processing order.  Entry i of a column covers
-// its slots [rptr + i * kRecvChunk, ...), at most kRecvChunk of them.
+// its slots [rptr + i * kRecvChunk, ...), at most kRecvChunk of them; with
+// `direct`, rows with a single remote entry have none.
 __global__ void k_receive_list(const int2 *__restrict__ need,
                                const int *__restrict__ rptr,
                                const int *__restrict__ pos,
                                const int *__restrict__ bid,
                                const int *__restrict__ shift,
                                const int *__restrict__ winPtr, int nwin, int n,
-                               int backward, int uniform_w, int4 *rlist,
-                               int *roff) {
+                               int backward, int uniform_w, int direct,
+                               int4 *rlist, int *roff) {
   const int stride = gridDim.x * blockDim.x;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
     const int c = backward ? n - 1 - p : p;
     const int m = need[c].y;
-    if (m > 0) {
+    if (m > (direct ? 1 : 0)) {
       const int w = uniform_w > 0 ? p / uniform_w
                                   : upper_bound_minus1(winPtr, nwin + 1, p);
       const int e0 = pos[p] + shift[bid[p]];

@@ -15,6 +15,7 @@ import weakref
 import numpy as np
 
 from ._lib import (COL_PARTITION, ENGINE_GROUP, ENGINE_LANE, ENGINE_ROW, EXPORTS, F32, F64,
+                   REMOTE_DIRECT, REMOTE_RECEIVER,
                    LIB_PATH, NNZ_PARTITION,
                    OPT_WARP_AUTO, OPT_WARP_NNZ, OPT_WARP_RHS, PLAN_MIXED,
                    PLAN_VALIDATE, STATUS, SUBSTITUTION_BACKWARD,
@@ -23,6 +24,7 @@ from ._lib import (COL_PARTITION, ENGINE_GROUP, ENGINE_LANE, ENGINE_ROW, EXPORTS
 
 __all__ = [
     "COL_PARTITION", "NNZ_PARTITION", "ENGINE_GROUP", "ENGINE_LANE", "ENGINE_ROW", "OPT_WARP_AUTO", "OPT_WARP_NNZ",
+    "REMOTE_RECEIVER", "REMOTE_DIRECT",
     "OPT_WARP_RHS", "SUBSTITUTION_BACKWARD", "SUBSTITUTION_FORWARD", "F32",
     "F64", "PLAN_MIXED", "PLAN_VALIDATE", "STATUS", "EXPORTS", "LIB_PATH",
     "Config", "SptrsvError", "lib", "check", "DeviceArray", "Plan", "Graph",

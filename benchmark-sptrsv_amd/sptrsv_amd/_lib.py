@@ -24,6 +24,8 @@ PLAN_MIXED = 2
 ENGINE_GROUP = 0
 ENGINE_LANE = 1
 ENGINE_ROW = 2
+REMOTE_RECEIVER = 1
+REMOTE_DIRECT = 2
 
 STATUS = {
     0: "OK",
@@ -80,6 +82,7 @@ class Config(ctypes.Structure):
         ("partition", ctypes.c_int),
         ("grid", ctypes.c_int),
         ("engine", ctypes.c_int),
+        ("remote", ctypes.c_int),
     ]
 
     def as_dict(self):

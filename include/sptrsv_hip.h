@@ -203,7 +203,18 @@ typedef struct {
   int partition;        /* SPTRSV_COL_PARTITION / SPTRSV_NNZ_PARTITION */
   int grid;             /* workgroups launched; 0 = resident capacity */
   int engine;           /* SPTRSV_ENGINE_GROUP, _LANE or _ROW */
+  int remote;           /* lane engine: SPTRSV_REMOTE_RECEIVER or _DIRECT
+                           (0 = choose; other engines: 0) */
 } sptrsv_config_t;
+
+/* How the lane engine delivers an entry from an earlier window.  RECEIVER:
+ * receiver wavefronts take the slots and add them into the row's LDS sum.
+ * DIRECT (one right-hand side per lane only): a row with exactly one such
+ * entry takes its slot in its own lane, one handoff less on the critical
+ * path; the default where windows hold whole grid periods (planes, rows,
+ * chains), receivers otherwise. */
+#define SPTRSV_REMOTE_RECEIVER 1
+#define SPTRSV_REMOTE_DIRECT 2
 
 /* Executor engines.  GROUP: lane groups of a wavefront per column, counters
  * and partial sums in LDS within a window and in HBM (atomics + fences)
