@@ -1869,13 +1869,27 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     fill_remote(p, &c);
     if (check_config(p, c, tsize) == SPTRSV_OK) out.push_back(c);
   };
+  // direct slots are timed only where the default would take them (windows
+  // of whole grid periods, chains): on a 128^3 grid with half-plane windows
+  // their solve times spread widely (0.35-0.40 ms) and a short timing can
+  // pick them over receivers that are faster on average
+  bool try_direct = false;
+  {
+    sptrsv_config_t d;
+    memset(&d, 0, sizeof(d));
+    d.engine = SPTRSV_ENGINE_LANE;
+    d.rhs_per_lane = 1;
+    d.window = lane_window(p, 1, tsize);
+    fill_remote(p, &d);
+    try_direct = d.remote == SPTRSV_REMOTE_DIRECT;
+  }
   for (int R : {1, 2, 4, 8}) {
     if (R > 1 && R >= 2 * rhs) break;  // no tile narrower than half of R
     // the period-aligned window, then the two widest powers of two that fit
     const int aligned = lane_window(p, R, tsize);
     for (int block : {1024, 512, 256})
       add(1, 1, R, block, aligned, 0, SPTRSV_ENGINE_LANE);
-    if (R == 1)  // both ways of taking entries from earlier windows
+    if (R == 1 && try_direct)  // both ways of taking remote entries
       for (int remote : {SPTRSV_REMOTE_RECEIVER, SPTRSV_REMOTE_DIRECT})
         add(1, 1, R, 1024, aligned, 0, SPTRSV_ENGINE_LANE, remote);
     // windows of balanced nonzero counts (the reference's equal_nnz_partition
@@ -1887,7 +1901,7 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
       if (window != aligned) {
         for (int block : {1024, 512, 256})
           add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
-        if (R == 1)
+        if (R == 1 && try_direct)
           for (int remote : {SPTRSV_REMOTE_RECEIVER, SPTRSV_REMOTE_DIRECT})
             add(1, 1, R, 1024, window, 0, SPTRSV_ENGINE_LANE, remote);
       }

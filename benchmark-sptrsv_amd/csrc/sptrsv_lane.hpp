@@ -331,7 +331,9 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < R; q++) {
-      const T a0 = wide_quot(s_val[base * R + q] - rem[q], d0, iv0);
+      T r0 = s_val[base * R + q];
+      if constexpr (DIRECT) r0 -= rem[q];
+      const T a0 = wide_quot(r0, d0, iv0);
       s_val[base * R + q] = a0;
       A[q] = __builtin_fma(P, a0, A[q]);
     }
@@ -503,9 +505,12 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     for (int q = 0; q < R; q++) xs[q] = T(0);
     // the coefficient half of the chain scan depends only on the factor: its
     // six steps are composed while the batch waits, for the first round
+    // (not with DIRECT, whose slot polls need those registers: about 1% on
+    // grids against spills)
+    constexpr bool kPre = !DIRECT;
     T pc[6];
     const unsigned long long act0 = __ballot(active);
-    if (chain_batch) {
+    if (kPre && chain_batch) {
       T c = active ? coef : T(0);
       pc[0] = c;
       c = c * dpp_t<0x111, 0xf>(T(1), c);
@@ -523,32 +528,27 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     // changes no lane can become solvable, so the round is not repeated
     unsigned long long stale = 0;
     StallWatch watch;
-    T rem[R];
-#pragma unroll
-    for (int q = 0; q < R; q++) rem[q] = T(0);
-    const unsigned rall = direct ? (1u << rv) - 1 : 0u;
-    unsigned rgot = 0;  // bit q: the remote value of right-hand side q taken
+    // DIRECT: the remote value taken (rem) and whether it is in (rgot)
+    T rem = T(0);
+    bool rgot = !direct;
     while (__ballot(active) != 0) {
-      // the slot polls are issued before the LDS count is read, so both
-      // round trips overlap
-      T sv[R];
-      const bool poll = DIRECT && active && rgot != rall;
-#pragma unroll
-      for (int q = 0; q < R; q++) {
-        sv[q] = empty;
-        if (poll && q < rv && !(rgot & (1u << q)))
-          sv[q] = xchg_take(&slots[(size_t)sidx * R + q], empty);
-      }
-      const int cnt = active ? ld_wg(&s_cnt[k]) : 0;
       bool took = false;
-#pragma unroll
-      for (int q = 0; q < R; q++)
-        if (!slot_empty(sv[q])) {
-          rem[q] = sv[q];
-          rgot |= 1u << q;
+      int cnt;
+      if constexpr (DIRECT) {
+        // the slot poll is issued before the LDS count is read, so both
+        // round trips overlap
+        T sv = empty;
+        if (active && !rgot) sv = xchg_take(&slots[sidx], empty);
+        cnt = active ? ld_wg(&s_cnt[k]) : 0;
+        if (!slot_empty(sv)) {
+          rem = sv;
+          rgot = true;
           took = true;
         }
-      const bool ext = active && cnt == need_lds && rgot == rall;
+      } else {
+        cnt = active ? ld_wg(&s_cnt[k]) : 0;
+      }
+      const bool ext = active && cnt == need_lds && (!DIRECT || rgot);
       bool progressed = false;  // some lane solved in this round
       const unsigned long long extm = __ballot(ext);
       if (extm != 0 && extm != stale) {
@@ -559,7 +559,8 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         T av[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
-          const T r = ext ? s_val[k * R + q] - rem[q] : T(0);
+          T r = ext ? s_val[k * R + q] : T(0);
+          if constexpr (DIRECT) r = ext ? r - rem : T(0);
           T qv = r * inv;
           qv = __builtin_fma(__builtin_fma(-qv, diag, r), inv, qv);
           av[q] = active ? qv : xs[q];
@@ -571,7 +572,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           // scan every cf is 0 and av is x wherever the segment is ready
           const unsigned long long linked = __ballot(cf != T(0));
           const unsigned long long ready = __ballot(ok);
-          if (__ballot(active) == act0) {
+          if (kPre && __ballot(active) == act0) {
             scan_step_pc<T, R, 0x111, 0xf>(av, pc[0]);
             scan_step_pc<T, R, 0x112, 0xf>(av, pc[1]);
             scan_step_pc<T, R, 0x114, 0xf>(av, pc[2]);
@@ -683,7 +684,8 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       // from delaying a solving wavefront, and a sleep would only delay the
       // poll that sees the next arrival.  A wavefront whose lanes have inputs
       // but wait on an unsolved predecessor counts as stalled.
-      if (watch.step(progressed || __ballot(took) != 0, a.err, a.wait_cap)) {
+      if constexpr (DIRECT) progressed = progressed || __ballot(took) != 0;
+      if (watch.step(progressed, a.err, a.wait_cap)) {
         active = false;
         stopped = true;
       }
