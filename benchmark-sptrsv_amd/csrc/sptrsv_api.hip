@@ -416,7 +416,8 @@ struct sptrsv_plan_s {
   // mixed precision; the CSR (values in the plan's type) also serves the row
   // engine, csrVal32 its FP32 solves of an FP64 plan
   DevBuf val32, csrPtr, csrIdx, csrVal, xtmp, r32, d32, csrVal32;
-  DevBuf farcnt;  // unsigned long long: entries >= kFarDistance off the diagonal
+  DevBuf farcnt;  // unsigned long long[2]: entries >= kFarDistance off the
+                  // diagonal, most entries in one row
   DevBuf lvlOrder;  // int[n]: solve positions sorted by DAG level
   DevBuf lvlRows;   // int4[n]: row engine's row table in level order
   bool csr_ready = false;
@@ -430,6 +431,7 @@ struct sptrsv_plan_s {
   double far_frac = 0.0;  // off-diagonal entries >= kFarDistance from the diagonal
   int period = 0;         // most common column reach (grid plane/row), or 0
   bool chain = false;     // most columns reach only the next row
+  int max_row = 0;        // most entries (diagonal included) in one row
   // captured graphs hold raw pointers to the scratch buffers: while any
   // exists, scratch is never reallocated (a larger request is refused)
   int live_graphs = 0;
@@ -457,13 +459,26 @@ int grow_scratch(sptrsv_plan_s *p, DevBuf &b, size_t want, bool zero) {
 // 5.8 ms with 8192-column windows against 0.32 ms with plane-aligned ones).
 // Batches of systems keep the power of two (their windows follow the
 // systems).
+// One right-hand side per lane can keep its LDS arrival counters in bytes
+// when no row has more than 255 entries; a window uses them only when it
+// does not fit in LDS with int counters (a whole plane of a 128^3 grid:
+// 0.36 -> 0.29 ms per FP64 solve), as extracting a byte costs the windows
+// that fit either way 1-2%.
+bool lane_cc_ok(const sptrsv_plan_s *p, int R) {
+  return R == 1 && p->max_row <= 255;
+}
+bool lane_cc(const sptrsv_plan_s *p, int R, int window, size_t tsize) {
+  return lane_cc_ok(p, R) && lane_lds_bytes(window, R, tsize) > p->lds_max;
+}
+
 int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
+  const bool cc = lane_cc_ok(p, R);
   int pow2 = 8192;
-  while (pow2 > 256 && lane_lds_bytes(pow2, R, tsize) > p->lds_max) pow2 /= 2;
+  while (pow2 > 256 && lane_lds_bytes(pow2, R, tsize, cc) > p->lds_max) pow2 /= 2;
   const int D = p->period;
   if (D <= pow2 || p->nsys > 1) return pow2;  // whole periods of <= 8192
   int wmax = 16384;  // the widest window that fits, a multiple of 64
-  while (wmax > 256 && lane_lds_bytes(wmax, R, tsize) > p->lds_max) wmax -= 64;
+  while (wmax > 256 && lane_lds_bytes(wmax, R, tsize, cc) > p->lds_max) wmax -= 64;
   if (D <= wmax) return D;  // one period per window
   const int k = (D + wmax - 1) / wmax;  // else each period split evenly
   return (D + k - 1) / k;
@@ -475,17 +490,22 @@ int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
 // chain), through receivers otherwise (windows that split a plane leave
 // rows with two remote entries, and the receivers busy; batches of
 // systems).  Other engines carry 0.
+bool direct_default(const sptrsv_plan_s *p, int R, int window) {
+  const bool whole = p->nsys == 1 &&
+                     ((p->period > 0 && p->period <= window) ||
+                      (p->period == 0 && p->chain));
+  return R == 1 && whole;
+}
+
 void fill_remote(const sptrsv_plan_s *p, sptrsv_config_t *c) {
   if (c->engine != SPTRSV_ENGINE_LANE) {
     c->remote = 0;
     return;
   }
   if (c->remote != 0) return;
-  const bool whole = p->nsys == 1 &&
-                     ((p->period > 0 && p->period <= c->window) ||
-                      (p->period == 0 && p->chain));
-  c->remote = c->rhs_per_lane == 1 && whole ? SPTRSV_REMOTE_DIRECT
-                                           : SPTRSV_REMOTE_RECEIVER;
+  c->remote = direct_default(p, c->rhs_per_lane, c->window)
+                  ? SPTRSV_REMOTE_DIRECT
+                  : SPTRSV_REMOTE_RECEIVER;
 }
 
 // Row-engine lane layouts compiled: (lanes per row, lanes over right-hand
@@ -571,7 +591,7 @@ int check_config(const sptrsv_plan_s *p, const sptrsv_config_t &c, int tsize) {
     if (c.partition != SPTRSV_COL_PARTITION &&
         c.partition != SPTRSV_NNZ_PARTITION)
       return SPTRSV_ERR_INVALID_ARG;
-    if (lane_lds_bytes(c.window, R, tsize) > p->lds_max)
+    if (lane_lds_bytes(c.window, R, tsize, lane_cc_ok(p, R)) > p->lds_max)
       return SPTRSV_ERR_INVALID_ARG;
     if (c.remote != SPTRSV_REMOTE_RECEIVER &&
         !(c.remote == SPTRSV_REMOTE_DIRECT && R == 1))
@@ -1278,26 +1298,33 @@ using LaneKernel = void (*)(LaneArgs<T>);
 // Lane-engine kernel for R right-hand sides per lane (1, 2, 4 or 8), with
 // or without the wide-batch path (compiled in only where the window set has
 // a wide batch: it costs the others a few registers); rows with one remote
-// entry take it directly only with one right-hand side per lane.
+// entry take it directly, and byte-wide arrival counters (cc), only with
+// one right-hand side per lane.
 template <typename T, bool BWD, bool WIDE>
-LaneKernel<T> pick_lane_r(int R, bool direct) {
+LaneKernel<T> pick_lane_r(int R, bool direct, bool cc) {
+  if (R == 1) {
+    if (cc)
+      return direct ? k_exec_lane<T, BWD, 1, WIDE, true, true>
+                    : k_exec_lane<T, BWD, 1, WIDE, false, true>;
+    return direct ? k_exec_lane<T, BWD, 1, WIDE, true, false>
+                  : k_exec_lane<T, BWD, 1, WIDE, false, false>;
+  }
+  if (direct || cc) return nullptr;
   switch (R) {
-    case 1: return direct ? k_exec_lane<T, BWD, 1, WIDE, true>
-                          : k_exec_lane<T, BWD, 1, WIDE, false>;
-    case 2: return direct ? nullptr : k_exec_lane<T, BWD, 2, WIDE, false>;
-    case 4: return direct ? nullptr : k_exec_lane<T, BWD, 4, WIDE, false>;
-    case 8: return direct ? nullptr : k_exec_lane<T, BWD, 8, WIDE, false>;
+    case 2: return k_exec_lane<T, BWD, 2, WIDE, false, false>;
+    case 4: return k_exec_lane<T, BWD, 4, WIDE, false, false>;
+    case 8: return k_exec_lane<T, BWD, 8, WIDE, false, false>;
     default: return nullptr;
   }
 }
 
 template <typename T>
-LaneKernel<T> pick_lane(bool bwd, int R, bool wide, bool direct) {
+LaneKernel<T> pick_lane(bool bwd, int R, bool wide, bool direct, bool cc) {
   if (bwd)
-    return wide ? pick_lane_r<T, true, true>(R, direct)
-                : pick_lane_r<T, true, false>(R, direct);
-  return wide ? pick_lane_r<T, false, true>(R, direct)
-              : pick_lane_r<T, false, false>(R, direct);
+    return wide ? pick_lane_r<T, true, true>(R, direct, cc)
+                : pick_lane_r<T, true, false>(R, direct, cc);
+  return wide ? pick_lane_r<T, false, true>(R, direct, cc)
+              : pick_lane_r<T, false, false>(R, direct, cc);
 }
 
 // Receiver wavefronts of a lane-engine workgroup: a quarter of them, at least
@@ -1313,6 +1340,7 @@ struct Launch {
   WindowSet *ws = nullptr;
   int ntiles = 0, R = 0, grid = 0;
   size_t lds = 0;
+  bool cc = false;  // lane engine: byte-wide arrival counters
   const void *rowval = nullptr;  // row engine: CSR values of the solve type
 };
 
@@ -1437,9 +1465,12 @@ int prepare_exec_impl(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     L->R = c.rhs_per_lane;
     L->ntiles = (rhs + L->R - 1) / L->R;
     // window rows plus one dummy row per lane, then the batch starts
-    L->lds = ((size_t)L->ws->maxw + 64) * ((size_t)L->R * sizeof(T) + sizeof(int)) +
+    const bool cc = lane_cc(p, L->R, c.window, sizeof(T));
+    L->cc = cc;
+    L->lds = ((size_t)L->ws->maxw + 64) * (size_t)L->R * sizeof(T) +
+             cnt_ints((size_t)L->ws->maxw + 64, cc) * sizeof(int) +
              2 * sizeof(int) * (size_t)L->ws->maxnb;
-    if (L->lds > lane_lds_bytes(L->ws->maxw, L->R, sizeof(T)))
+    if (L->lds > lane_lds_bytes(L->ws->maxw, L->R, sizeof(T), cc))
       return SPTRSV_ERR_INVALID_ARG;  // cannot happen: maxnb <= maxw/16 + 1
     DevBuf &sb = slot_buf<T>(p);
     const size_t want =
@@ -1452,7 +1483,7 @@ int prepare_exec_impl(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
     }
     LaneKernel<T> kern =
         pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L->R,
-                     L->ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT);
+                     L->ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT, cc);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     if (L->lds > 65536)
       HIP_TRY(hipFuncSetAttribute((const void *)kern,
@@ -1580,7 +1611,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
     LaneKernel<T> kern =
         pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L.R,
-                     L.ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT);
+                     L.ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT, L.cc);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     kern<<<L.grid, c.block, L.lds, s>>>(la);
     HIP_TRY(hipGetLastError());
@@ -1704,7 +1735,7 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
   if (st == SPTRSV_OK && (hipEventCreate(&e0) != hipSuccess ||
                           hipEventCreate(&e1) != hipSuccess))
     st = SPTRSV_ERR_HIP;
-  unsigned long long far_entries = 0;
+  unsigned long long far_stats[2] = {0, 1};  // far entries, longest row
   if (st == SPTRSV_OK) st = p->sync.ensure(kSyncWords * sizeof(int), true);
   if (st == SPTRSV_OK) st = p->inDegree.ensure(sizeof(int) * std::max(n, 1), false);
   if (st == SPTRSV_OK && hipEventRecord(e0, s) != hipSuccess) st = SPTRSV_ERR_HIP;
@@ -1754,13 +1785,17 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
         p->chain = 5 * n1 >= 2 * nsamp;
       }
     }
-    // entries farther from the diagonal than the default lane window
-    if (st == SPTRSV_OK && nnz > n) st = p->farcnt.ensure(sizeof(unsigned long long), true);
+    // entries farther from the diagonal than the default lane window, and
+    // the longest row
+    if (st == SPTRSV_OK && nnz > n)
+      st = p->farcnt.ensure(2 * sizeof(unsigned long long), true);
     if (st == SPTRSV_OK && nnz > n) {
-      k_count_far<<<std::min(grid_for(n, 256), 2048), 256, 0, s>>>(
+      k_count_far<<<std::min(grid_for(n, 256), 256), 256, 0, s>>>(
           colPtr, rowIdx, n, kFarDistance, p->farcnt.as<unsigned long long>());
+      k_max_int<<<std::min(grid_for(n, 256), 256), 256, 0, s>>>(
+          p->inDegree.as<int>(), n, p->farcnt.as<unsigned long long>() + 1);
       if (hipGetLastError() != hipSuccess ||
-          hipMemcpyAsync(&far_entries, p->farcnt.p, sizeof(far_entries),
+          hipMemcpyAsync(far_stats, p->farcnt.p, sizeof(far_stats),
                          hipMemcpyDeviceToHost, s) != hipSuccess)
         st = SPTRSV_ERR_HIP;
     }
@@ -1785,7 +1820,8 @@ int plan_create_impl(sptrsv_plan_t *out, int n, int nnz, const int *colPtr,
   if (st == SPTRSV_OK && hipEventRecord(e1, s) != hipSuccess) st = SPTRSV_ERR_HIP;
   if (st == SPTRSV_OK && hipEventSynchronize(e1) != hipSuccess) st = SPTRSV_ERR_HIP;
   if (st == SPTRSV_OK) {
-    p->far_frac = nnz > n ? (double)far_entries / (double)(nnz - n) : 0.0;
+    p->far_frac = nnz > n ? (double)far_stats[0] / (double)(nnz - n) : 0.0;
+    p->max_row = (int)std::min<unsigned long long>(far_stats[1], 1u << 30);
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     p->analysis_ms = ms;
@@ -1873,23 +1909,13 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
   // of whole grid periods, chains): on a 128^3 grid with half-plane windows
   // their solve times spread widely (0.35-0.40 ms) and a short timing can
   // pick them over receivers that are faster on average
-  bool try_direct = false;
-  {
-    sptrsv_config_t d;
-    memset(&d, 0, sizeof(d));
-    d.engine = SPTRSV_ENGINE_LANE;
-    d.rhs_per_lane = 1;
-    d.window = lane_window(p, 1, tsize);
-    fill_remote(p, &d);
-    try_direct = d.remote == SPTRSV_REMOTE_DIRECT;
-  }
   for (int R : {1, 2, 4, 8}) {
     if (R > 1 && R >= 2 * rhs) break;  // no tile narrower than half of R
     // the period-aligned window, then the two widest powers of two that fit
     const int aligned = lane_window(p, R, tsize);
     for (int block : {1024, 512, 256})
       add(1, 1, R, block, aligned, 0, SPTRSV_ENGINE_LANE);
-    if (R == 1 && try_direct)  // both ways of taking remote entries
+    if (direct_default(p, R, aligned))  // both ways of taking remote entries
       for (int remote : {SPTRSV_REMOTE_RECEIVER, SPTRSV_REMOTE_DIRECT})
         add(1, 1, R, 1024, aligned, 0, SPTRSV_ENGINE_LANE, remote);
     // windows of balanced nonzero counts (the reference's equal_nnz_partition
@@ -1897,11 +1923,12 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     add(1, 1, R, 1024, aligned, SPTRSV_NNZ_PARTITION, SPTRSV_ENGINE_LANE);
     int added = 0;
     for (int window : {8192, 4096, 2048, 1024}) {
-      if (lane_lds_bytes(window, R, tsize) > p->lds_max) continue;
+      if (lane_lds_bytes(window, R, tsize, lane_cc_ok(p, R)) > p->lds_max)
+        continue;
       if (window != aligned) {
         for (int block : {1024, 512, 256})
           add(1, 1, R, block, window, 0, SPTRSV_ENGINE_LANE);
-        if (R == 1 && try_direct)
+        if (direct_default(p, R, window))
           for (int remote : {SPTRSV_REMOTE_RECEIVER, SPTRSV_REMOTE_DIRECT})
             add(1, 1, R, 1024, window, 0, SPTRSV_ENGINE_LANE, remote);
       }

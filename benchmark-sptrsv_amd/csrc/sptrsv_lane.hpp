@@ -170,12 +170,36 @@ constexpr int kRegTarget = -2147483647 - 1;
 // many columns (so a window has at most window / kMinChainBatch + 1 batches).
 constexpr int kMinChainBatch = 16;
 
+// LDS arrival counters of `rows` rows: one int each, or, compact (CC), one
+// byte each, four to an int -- a row's count never exceeds its off-diagonal
+// entries, so a byte holds it when no row has more than 255 entries.
+// Compact counters let a window of one right-hand side hold 16,384 FP64
+// rows in LDS (a whole plane of a 128^3 grid) instead of 13,600.
+__host__ __device__ constexpr size_t cnt_ints(size_t rows, bool cc) {
+  return cc ? (rows + 3) / 4 : rows;
+}
+template <bool CC>
+__device__ __forceinline__ int cnt_load(const int *c, int k) {
+  if constexpr (CC)
+    return (int)(((unsigned)ld_wg(&c[k >> 2]) >> (8 * (k & 3))) & 0xffu);
+  else
+    return ld_wg(&c[k]);
+}
+template <bool CC>
+__device__ __forceinline__ void cnt_add(int *c, int k, int v) {
+  if constexpr (CC)
+    add_wg(&c[k >> 2], (int)((unsigned)v << (8 * (k & 3))));
+  else
+    add_wg(&c[k], v);
+}
+
 // LDS of a lane-engine workgroup: window rows plus one dummy row per lane
 // (R values and a counter each), then the window's batch starts and its
 // batch processing order.
 __host__ __device__ inline size_t lane_lds_bytes(size_t window, int R,
-                                                 size_t tsize) {
-  return (window + kWave) * ((size_t)R * tsize + sizeof(int)) +
+                                                 size_t tsize, bool cc = false) {
+  return (window + kWave) * (size_t)R * tsize +
+         cnt_ints(window + kWave, cc) * sizeof(int) +
          2 * sizeof(int) * (window / kMinChainBatch + 2);
 }
 
@@ -215,15 +239,15 @@ __device__ __forceinline__ void wide_publish(const LaneArgs<T> &a, T *s_val,
 }
 
 // ... and then counts its LDS arrivals.
-template <typename T>
+template <typename T, bool CC>
 __device__ __forceinline__ void wide_count(const LaneArgs<T> &a, int *s_cnt,
                                            const int (&tg)[kPF], int lo, int hi) {
 #pragma unroll
   for (int i = 0; i < kPF; i++)
-    if (tg[i] < 0 && tg[i] != kRegTarget) add_wg(&s_cnt[-tg[i] - 1], 1);
+    if (tg[i] < 0 && tg[i] != kRegTarget) cnt_add<CC>(s_cnt, -tg[i] - 1, 1);
   for (int j = lo + kPF; j < hi; j++) {
     const int tj = a.etgt[j];
-    if (tj < 0 && tj != kRegTarget) add_wg(&s_cnt[-tj - 1], 1);
+    if (tj < 0 && tj != kRegTarget) cnt_add<CC>(s_cnt, -tj - 1, 1);
   }
 }
 
@@ -240,7 +264,7 @@ __device__ __forceinline__ void wide_count(const LaneArgs<T> &a, int *s_cnt,
 // the batch's last column (the last of lane `owner`) publishes at once, and
 // then each lane expands its columns from its predecessor lane's last x.
 // Returns false when the stall watch fired.
-template <typename T, bool BWD, int R, bool DIRECT>
+template <typename T, bool BWD, int R, bool DIRECT, bool CC>
 __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
                                           int *s_cnt, int base, int bend,
                                           int wlo, int tile, int rv) {
@@ -324,7 +348,7 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
     }
   }
   StallWatch watch;
-  while (ld_wg(&s_cnt[base]) != need0)
+  while (cnt_load<CC>(s_cnt, base) != need0)
     if (watch.step(false, a.err, a.wait_cap)) return false;
   __builtin_amdgcn_s_setprio(3);
   fence_acq_wg();
@@ -345,7 +369,7 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
     for (int q = 0; q < rv; q++)
       wide_publish<T, R>(a, s_val, slots, tg, vv, lo, hi, A[q], q);
   fence_rel_wg();
-  if (lane == owner) wide_count(a, s_cnt, tg, lo, hi);
+  if (lane == owner) wide_count<T, CC>(a, s_cnt, tg, lo, hi);
   __builtin_amdgcn_s_setprio(0);
   T x[R];
 #pragma unroll
@@ -388,10 +412,12 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
 // than through a receiver's LDS deposit and counter, one handoff less on a
 // cross-window step.  It pays where windows hold whole grid planes, so that
 // nearly every row is such a row (a 100^3 grid: 0.34 -> 0.23 ms, the 10^6
-// chain's wide batches 0.46 -> 0.36 ms); where half-plane windows leave the
-// receivers busy too, the extra pollers cost more than the handoff saves
-// (128^3: +6%), and the configuration chooses.
-template <typename T, bool BWD, int R, bool WIDE, bool DIRECT>
+// chain's wide batches 0.46 -> 0.36 ms; a 128^3 grid, whose planes fit with
+// byte-wide counters, 0.36 -> 0.29 ms); where windows split a plane and
+// leave the receivers busy too, the extra pollers cost more than the
+// handoff saves (128^3 with half-plane windows: +6%), and the configuration
+// chooses.
+template <typename T, bool BWD, int R, bool WIDE, bool DIRECT, bool CC>
 __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, const int *s_bst,
                                             const int *s_bord, int nb, int wlo,
@@ -414,7 +440,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     const int bend = bi + 1 < nb ? s_bst[bi + 1] : wlen;
     if constexpr (WIDE) {
       if (bend - base > kWave) {  // a run of pure links: one scan
-        if (!lane_wide<T, BWD, R, DIRECT>(a, s_val, s_cnt, base, bend, wlo, tile, rv))
+        if (!lane_wide<T, BWD, R, DIRECT, CC>(a, s_val, s_cnt, base, bend, wlo, tile, rv))
           stopped = true;
         continue;
       }
@@ -539,14 +565,14 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         // round trips overlap
         T sv = empty;
         if (active && !rgot) sv = xchg_take(&slots[sidx], empty);
-        cnt = active ? ld_wg(&s_cnt[k]) : 0;
+        cnt = active ? cnt_load<CC>(s_cnt, k) : 0;
         if (!slot_empty(sv)) {
           rem = sv;
           rgot = true;
           took = true;
         }
       } else {
-        cnt = active ? ld_wg(&s_cnt[k]) : 0;
+        cnt = active ? cnt_load<CC>(s_cnt, k) : 0;
       }
       const bool ext = active && cnt == need_lds && (!DIRECT || rgot);
       bool progressed = false;  // some lane solved in this round
@@ -638,7 +664,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           if (solved) {
 #pragma unroll
             for (int e = 0; e < kPF; e++)
-              add_wg(&s_cnt[ladr[e]], linc[e]);
+              cnt_add<CC>(s_cnt, ladr[e], linc[e]);
           }
           // entries past the registers, kLong at a time: loads issued
           // together, one wait per group; each group's LDS values are
@@ -674,7 +700,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 #pragma unroll
               for (int i = 0; i < kLong; i++)
                 if (tt[i] < 0 && tt[i] != kRegTarget)
-                  add_wg(&s_cnt[-tt[i] - 1], 1);
+                  cnt_add<CC>(s_cnt, -tt[i] - 1, 1);
             }
           }
         }
@@ -707,7 +733,7 @@ constexpr int kTake = 8;  // slots of one row taken per group
 // several lanes at once and added to the row's LDS sum (and count) one by
 // one.  A receiver
 // wavefront that took nothing in an iteration backs off.
-template <typename T, int R>
+template <typename T, int R, bool CC>
 __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
                                             int *s_cnt, int win, int tile,
                                             int rv, int rw) {
@@ -767,7 +793,7 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
         if (s == r1) {
           add_wg(&s_val[k * R + q], -sum);
           fence_rel_wg();
-          add_wg(&s_cnt[k], m);
+          cnt_add<CC>(s_cnt, k, m);
           done = true;
         }
       }
@@ -787,14 +813,14 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
 // per lane need the register budget of 512-thread workgroups.
 constexpr int lane_max_block(int R) { return R >= 8 ? 512 : 1024; }
 
-template <typename T, bool BWD, int R, bool WIDE, bool DIRECT>
+template <typename T, bool BWD, int R, bool WIDE, bool DIRECT, bool CC>
 __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_val = reinterpret_cast<T *>(smem_raw);
   // maxw window rows plus 64 dummy rows (one per lane) for branch-free
   // publishing, then the window's batch starts
   int *s_cnt = reinterpret_cast<int *>(s_val + (size_t)(a.maxw + kWave) * R);
-  int *s_bst = s_cnt + a.maxw + kWave;
+  int *s_bst = s_cnt + cnt_ints(a.maxw + kWave, CC);
   int *s_bord = s_bst + a.maxnb;
   __shared__ int s_ticket;
   const int total = a.nwin * a.ntiles;
@@ -821,7 +847,8 @@ __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) 
       const int c = BWD ? n - 1 - p : p;
       s_val[i] = q < rv ? a.b[(size_t)c * a.rhs + tile * R + q] : T(0);
     }
-    for (int k = threadIdx.x; k < wlen; k += blockDim.x) s_cnt[k] = 0;
+    for (int k = threadIdx.x; k < (int)cnt_ints(wlen, CC); k += blockDim.x)
+      s_cnt[k] = 0;
     const int b0 = a.boff[win], nb = a.boff[win + 1] - b0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
       s_bst[i] = a.bstart[b0 + i];
@@ -829,9 +856,9 @@ __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) 
     }
     __syncthreads();
     if (wave < a.nrecv)
-      lane_receive<T, R>(a, s_val, s_cnt, win, tile, rv, wave);
+      lane_receive<T, R, CC>(a, s_val, s_cnt, win, tile, rv, wave);
     else
-      lane_compute<T, BWD, R, WIDE, DIRECT>(a, s_val, s_cnt, s_bst, s_bord, nb, wlo,
+      lane_compute<T, BWD, R, WIDE, DIRECT, CC>(a, s_val, s_cnt, s_bst, s_bord, nb, wlo,
                                     wlen, tile, rv, wave - a.nrecv, ncw);
     __syncthreads();
     for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {

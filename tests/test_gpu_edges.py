@@ -222,16 +222,48 @@ def test_batch_claim_order_largest_first(backward):
 def test_default_windows_follow_the_grid_period():
     # the default lane windows hold whole planes (or rows) of a grid: planes
     # up to 8192 columns are packed whole into 8192-column windows; a wider
-    # plane gets a window of exactly one plane when it fits in LDS, else it
-    # is split evenly (16384 -> 2 x 8192, 127^2 = 16129 -> 8065 + 8064)
+    # plane gets a window of exactly one plane when it fits in LDS (FP64
+    # planes of up to 16384 columns, with byte-wide arrival counters), else
+    # it is split evenly (130^2 = 16900 -> 2 x 8450)
     for dims, window in (((60, 50, 30), 8192), ((100, 100, 12), 10000),
-                         ((128, 128, 8), 8192), ((127, 127, 9), 8065)):
+                         ((128, 128, 8), 16384), ((127, 127, 9), 16129),
+                         ((130, 130, 3), 8450)):
         f = mx.laplacian3d_lower(*dims)
         plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
         assert plan.get_config(1)["window"] == window, dims
         b = f.spmv(mx.x_uniform(f.n, seed=3))
         x, _ = plan_solve(f, b, plan=plan)
         assert_close(x, oracle_x(f, b), 1e-12)
+
+
+def _lap3d_with_long_row(nx, ny, nz, extra):
+    """The 7-point Laplacian with `extra` more entries in its last row: half
+    from the first plane (entries from earlier windows), half from the last
+    plane (entries inside the row's own window)."""
+    f = mx.laplacian3d_lower(nx, ny, nz)
+    n, plane = f.n, nx * ny
+    cols = np.repeat(np.arange(n), np.diff(f.colPtr))
+    half = extra // 2
+    xc = np.r_[np.arange(half), n - plane + np.arange(extra - half)]
+    cols = np.r_[cols, xc]
+    rows = np.r_[f.rowIdx, np.full(extra, n - 1)]
+    vals = np.r_[f.val, np.full(extra, -1.0 / extra)]
+    cp, ri, vv = mx._from_columns(n, cols, rows, vals)
+    return mx.Factor(cp, ri, vv, name=f"lap3d_long{extra}")
+
+
+@pytest.mark.parametrize("extra,window", [(250, 16384), (300, 8192)])
+def test_byte_counters_follow_the_longest_row(extra, window):
+    # byte-wide LDS arrival counters hold counts up to 255: a plane of
+    # 16384 columns is one window while every row has at most 255 entries
+    # (here the last row has 254 and its count climbs to 253), and is split
+    # when one row has more
+    f = _lap3d_with_long_row(128, 128, 3, extra)
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val)
+    assert plan.get_config(1)["window"] == window
+    b = f.spmv(mx.x_uniform(f.n, seed=4))
+    x, _ = plan_solve(f, b, plan=plan)
+    assert_close(x, oracle_x(f, b), 1e-12)
 
 
 @pytest.mark.parametrize("backward", [0, 1])
