@@ -825,9 +825,11 @@ __device__ __forceinline__ void lane_receive(const LaneArgs<T> &a, T *s_val,
   }
 }
 
-// Largest lane-engine workgroup for R right-hand sides per lane: 8 values
-// per lane need the register budget of 512-thread workgroups.
-constexpr int lane_max_block(int R) { return R >= 8 ? 512 : 1024; }
+// Largest lane-engine workgroup for R right-hand sides per lane: 2 or more
+// values per lane need the register budget of 512-thread workgroups (with
+// 1024 threads, 128 VGPRs, the FP64 kernels spilled 2-4 registers at R = 2
+// and 20 at R = 4).
+constexpr int lane_max_block(int R) { return R >= 2 ? 512 : 1024; }
 
 template <typename T, bool BWD, int R, bool WIDE, bool DIRECT, bool CC>
 __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) {

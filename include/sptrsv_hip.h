@@ -220,20 +220,25 @@ typedef struct {
  * and partial sums in LDS within a window and in HBM (atomics + fences)
  * across windows -- all rhs modes.  LANE (the default): one lane per column
  * carrying rhs_per_lane right-hand sides of a tile, chains inside a
- * 64-column batch solved by a wavefront scan, compute wavefronts that touch
- * only LDS, and one value-carrying slot per cross-window entry and
- * right-hand side polled by receiver wavefronts (lanes_per_column =
- * rhs_lanes = 1, block >= 128; LDS = window * (rhs_per_lane * value size + 4)
- * bytes). */
+ * 64-column batch solved by a wavefront scan, and one value-carrying slot
+ * per cross-window entry and right-hand side, polled by receiver wavefronts
+ * or (remote = SPTRSV_REMOTE_DIRECT) by the row's own lane
+ * (lanes_per_column = rhs_lanes = 1; block 128..1024, at most 512 for
+ * rhs_per_lane >= 2; LDS ~ window * (rhs_per_lane * value size + 4) bytes,
+ * or + 1 with one right-hand side per lane when 4-byte counters would not
+ * fit and no row has more than 255 entries). */
 #define SPTRSV_ENGINE_GROUP 0
 #define SPTRSV_ENGINE_LANE 1
 /* ROW (no counterpart in the reference, whose executor only pushes): pulls
- * over the CSR of the factor, lanes_per_column (4, 16 or 64) lanes per row
- * and right-hand side reading each entry's solution, which is its own
+ * over the CSR of the factor, lanes_per_column (4, 16, 32 or 64) lanes per
+ * row and right-hand side reading each entry's solution, which is its own
  * arrival flag (x is filled with a sentinel first), so no atomics accumulate
- * and a row fed by hundreds of columns costs no serialised updates.  For
- * wide, shallow factors (random power-law rows); window and partition are
- * unused and the grid never exceeds the resident capacity. */
+ * and a row fed by hundreds of columns costs no serialised updates.  Rows
+ * are claimed in DAG-level order through a ticket, window rounds of the
+ * workgroup's row groups per claim, so any grid works.  With rhs_lanes =
+ * lanes_per_column (8..64) the lanes run over right-hand sides instead
+ * (rhs_per_lane 1, 2 or 4 each).  For wide, shallow factors (random
+ * power-law rows) and many right-hand sides; partition is unused. */
 #define SPTRSV_ENGINE_ROW 2
 
 #define SPTRSV_PLAN_VALIDATE 1 /* check the factor at creation */

@@ -214,7 +214,7 @@ RHS_CONFIGS = [None,
                dict(lanes_per_column=4, rhs_lanes=1, rhs_per_lane=4, block=256, window=64),
                dict(engine=S.ENGINE_LANE, block=256, window=512),
                dict(engine=S.ENGINE_LANE, rhs_per_lane=2, block=512, window=1024),
-               dict(engine=S.ENGINE_LANE, rhs_per_lane=4, block=1024, window=2048),
+               dict(engine=S.ENGINE_LANE, rhs_per_lane=4, block=512, window=2048),
                dict(engine=S.ENGINE_LANE, rhs_per_lane=8, block=256, window=300)]
 
 
