@@ -515,16 +515,15 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       }
     }
     // LDS target of each register entry; entries that do not go to LDS add
-    // 0 to this lane's own dummy row (past the window), so the publish is
-    // straight-line code with no per-entry branches
+    // their (unused) value and a count of 0 to this lane's own dummy row
+    // past the window, so the publish is straight-line code with no
+    // per-entry branches
     int ladr[kPF];
-    T lval[kPF];
     int linc[kPF];
 #pragma unroll
     for (int i = 0; i < kPF; i++) {
       const bool loc = i < ne && tg[i] < 0 && tg[i] != kRegTarget;
       ladr[i] = loc ? -tg[i] - 1 : a.maxw + lane;
-      lval[i] = loc ? vv[i] : T(0);
       linc[i] = loc ? 1 : 0;
     }
     // every predecessor link is lane - 1 (stencil rows, chains): scan with
@@ -536,9 +535,9 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     for (int q = 0; q < R; q++) xs[q] = T(0);
     // the coefficient half of the chain scan depends only on the factor: its
     // six steps are composed while the batch waits, for the first round
-    // (not with DIRECT, whose slot polls need those registers: about 1% on
-    // grids against spills)
-    constexpr bool kPre = !DIRECT;
+    // (2D grids -3%, 3D -1%; not in the kernels with wide batches and
+    // direct slots, whose chains it made 4% slower)
+    constexpr bool kPre = !(WIDE && DIRECT);
     T pc[6];
     const unsigned long long act0 = __ballot(active);
     if (kPre && chain_batch) {
@@ -672,7 +671,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
             for (int e = 0; e < kPF; e++)
 #pragma unroll
               for (int q = 0; q < R; q++)
-                add_wg(&s_val[ladr[e] * R + q], -(av[q] * lval[e]));
+                add_wg(&s_val[ladr[e] * R + q], -(av[q] * vv[e]));
           }
           // the register entries' rows (the next grid row of a stencil) are
           // released before the entries past the registers are loaded
