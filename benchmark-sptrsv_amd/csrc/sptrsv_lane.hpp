@@ -464,6 +464,13 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
     if (active) {
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
+      // without wide batches b joins the column's LDS sum here (the window
+      // starts from zeros): its load is issued first and lands with the
+      // diagonal's
+      T bq[R];
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        bq[q] = !WIDE && q < rv ? a.b[(size_t)c * a.rhs + tile * R + q] : T(0);
       int f, l, rp;
       int2 nd, ch;
       if (pf_bi == bi) { f = pf_f; l = pf_l; nd = pf_nd; ch = pf_ch; rp = pf_rp; }
@@ -475,6 +482,9 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       sidx = rp;
       diag = a.val[BWD ? l - 1 : f];
       inv = T(1) / diag;  // off the critical path
+      if constexpr (!WIDE)
+#pragma unroll
+        for (int q = 0; q < R; q++) add_wg(&s_val[k * R + q], bq[q]);
       lo = BWD ? f : f + 1;
       hi = BWD ? l - 1 : l;
       ne = min(hi - lo, kPF);
@@ -857,12 +867,21 @@ __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) 
     const int rv = min(R, a.rhs - tile * R);  // valid right-hand sides
     const int wlo = a.winPtr[win];
     const int wlen = a.winPtr[win + 1] - wlo;
+    // Partial sums start at zero and each column's b is added by its own
+    // lane when its batch starts: staging a 128^3 grid's plane of b here
+    // held the window's first batch back ~7 us (-2.6% on that grid).  Wide
+    // batches (pure chains) read b from LDS before their wait, so windows
+    // that have them stage b here (per batch it cost the 10^6 chain 3-5%).
     for (int i = threadIdx.x; i < wlen * R; i += blockDim.x) {
-      const int k = R == 1 ? i : i / R;
-      const int q = R == 1 ? 0 : i - k * R;
-      const int p = wlo + k;
-      const int c = BWD ? n - 1 - p : p;
-      s_val[i] = q < rv ? a.b[(size_t)c * a.rhs + tile * R + q] : T(0);
+      T v = T(0);
+      if constexpr (WIDE) {
+        const int k = R == 1 ? i : i / R;
+        const int q = R == 1 ? 0 : i - k * R;
+        const int p = wlo + k;
+        const int c = BWD ? n - 1 - p : p;
+        if (q < rv) v = a.b[(size_t)c * a.rhs + tile * R + q];
+      }
+      s_val[i] = v;
     }
     for (int k = threadIdx.x; k < (int)cnt_ints(wlen, CC); k += blockDim.x)
       s_cnt[k] = 0;
