@@ -451,14 +451,15 @@ int grow_scratch(sptrsv_plan_s *p, DevBuf &b, size_t want, bool zero) {
 }
 
 // Lane-engine window (the largest window size) for R right-hand sides: the
-// widest power of two up to 8192 that fits in LDS, or, for a period (grid
+// widest power of two up to 8192 (16384 for batches of systems) that fits in
+// LDS, or, for a period (grid
 // plane) wider than that, one period when it fits in LDS.  The windows
 // themselves follow the period (get_windows): whole periods, or each period
 // split evenly, so that no window holds the tail of one plane and the head
 // of the next -- such a window waits on both at once (a 100^3 grid took
 // 5.8 ms with 8192-column windows against 0.32 ms with plane-aligned ones).
 // Batches of systems keep the power of two (their windows follow the
-// systems).
+// systems; a system wider than a window is cut into window-size pieces).
 // One right-hand side per lane can keep its LDS arrival counters in bytes
 // when no row has more than 255 entries; a window uses them only when it
 // does not fit in LDS with int counters (a whole plane of a 128^3 grid:
@@ -473,7 +474,9 @@ bool lane_cc(const sptrsv_plan_s *p, int R, int window, size_t tsize) {
 
 int lane_window(const sptrsv_plan_s *p, int R, int tsize) {
   const bool cc = lane_cc_ok(p, R);
-  int pow2 = 8192;
+  // batches of systems: up to 16384 columns (fewer windows, fewer systems
+  // split between two: 512 uneven 2D systems 0.36 -> 0.27 ms)
+  int pow2 = p->nsys > 1 ? 16384 : 8192;
   while (pow2 > 256 && lane_lds_bytes(pow2, R, tsize, cc) > p->lds_max) pow2 /= 2;
   const int D = p->period;
   if (D <= pow2 || p->nsys > 1) return pow2;  // whole periods of <= 8192
@@ -1922,7 +1925,7 @@ std::vector<sptrsv_config_t> tune_candidates(const sptrsv_plan_s *p, int rhs,
     // as one GPU's window ordering) at the same maximum width
     add(1, 1, R, 1024, aligned, SPTRSV_NNZ_PARTITION, SPTRSV_ENGINE_LANE);
     int added = 0;
-    for (int window : {8192, 4096, 2048, 1024}) {
+    for (int window : {16384, 8192, 4096, 2048, 1024}) {
       if (lane_lds_bytes(window, R, tsize, lane_cc_ok(p, R)) > p->lds_max)
         continue;
       if (window != aligned) {

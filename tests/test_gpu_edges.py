@@ -219,6 +219,24 @@ def test_batch_claim_order_largest_first(backward):
         assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
 
 
+@pytest.mark.parametrize("backward", [0, 1])
+def test_batch_default_windows_16384(backward):
+    # batches of systems default to 16384-column windows (byte counters make
+    # room); systems wider than that (150^2 = 22500) are cut into pieces
+    f = mx.laplacian2d_batch(6, lo=120, hi=180, seed=3)
+    assert (np.diff(f.sys_ptr) > 16384).any() and (np.diff(f.sys_ptr) < 16384).any()
+    if backward:
+        sys_ptr = f.sys_ptr
+        f = mx.upper_from_lower(f)
+        f.sys_ptr = sys_ptr
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val, substitution=backward,
+                            sys_ptr=f.sys_ptr)
+    assert plan.get_config(1)["window"] == 16384
+    b = f.spmv(mx.x_uniform(f.n, seed=5))
+    x, _ = plan_solve(f, b, subst=backward, plan=plan)
+    assert_close(x, oracle_x(f, b, subst=backward), 1e-12)
+
+
 def test_default_windows_follow_the_grid_period():
     # the default lane windows hold whole planes (or rows) of a grid: planes
     # up to 8192 columns are packed whole into 8192-column windows; a wider
