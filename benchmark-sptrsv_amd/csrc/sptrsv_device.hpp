@@ -56,6 +56,15 @@ template <typename T>
 __device__ __forceinline__ void add_agent(T *p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Fused multiply-add in the operands' own precision: __builtin_fma is the
+// double routine, which would run a float kernel's arithmetic in FP64 with
+// conversions around every call.
+__device__ __forceinline__ double fma_t(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fma_t(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
 template <typename T>
 __device__ __forceinline__ T ld_wg(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

@@ -121,7 +121,7 @@ __device__ __forceinline__ void scan_step(T (&a)[R], T &c) {
   for (int q = 0; q < R; q++) a1[q] = dpp_t<CTRL, ROWMASK>(T(0), a[q]);
   const T c1 = dpp_t<CTRL, ROWMASK>(T(1), c);
 #pragma unroll
-  for (int q = 0; q < R; q++) a[q] = __builtin_fma(c, a1[q], a[q]);
+  for (int q = 0; q < R; q++) a[q] = fma_t(c, a1[q], a[q]);
   c = c * c1;
 }
 
@@ -130,7 +130,7 @@ __device__ __forceinline__ void scan_step(T (&a)[R], T &c) {
 template <typename T, int R, int CTRL, int ROWMASK>
 __device__ __forceinline__ void scan_step_pc(T (&a)[R], T c) {
 #pragma unroll
-  for (int q = 0; q < R; q++) a[q] = __builtin_fma(c, dpp_t<CTRL, ROWMASK>(T(0), a[q]), a[q]);
+  for (int q = 0; q < R; q++) a[q] = fma_t(c, dpp_t<CTRL, ROWMASK>(T(0), a[q]), a[q]);
 }
 
 // Inclusive scan over the 64 lanes: row_shr 1, 2, 4, 8 inside each row of
@@ -216,7 +216,7 @@ constexpr int kWideC = 32;
 template <typename T>
 __device__ __forceinline__ T wide_quot(T r, T d, T inv) {
   const T qv = r * inv;
-  return __builtin_fma(__builtin_fma(-qv, d, r), inv, qv);
+  return fma_t(fma_t(-qv, d, r), inv, qv);
 }
 
 // The last column of a wide batch publishes right-hand side q of its x:
@@ -307,7 +307,7 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
       for (int q = 0; q < R; q++) {
         const T qv = wide_quot(s_val[(k0 + m) * R + q], d, inv);
         s_val[(k0 + m) * R + q] = qv;
-        A[q] = __builtin_fma(cm, A[q], qv);
+        A[q] = fma_t(cm, A[q], qv);
       }
     } else {
       d0 = d;
@@ -364,7 +364,7 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
       if constexpr (DIRECT) r0 -= rem[q];
       const T a0 = wide_quot(r0, d0, iv0);
       s_val[base * R + q] = a0;
-      A[q] = __builtin_fma(P, a0, A[q]);
+      A[q] = fma_t(P, a0, A[q]);
     }
   }
   chain_scan<T, R>(A, K);  // lane 0's K is 0: every A is now absolute
@@ -387,7 +387,7 @@ __device__ __forceinline__ bool lane_wide(const LaneArgs<T> &a, T *s_val,
     const T cm = coef(k0 + m, &d, &inv);
 #pragma unroll
     for (int q = 0; q < R; q++) {
-      x[q] = __builtin_fma(cm, x[q], s_val[(k0 + m) * R + q]);
+      x[q] = fma_t(cm, x[q], s_val[(k0 + m) * R + q]);
       s_val[(k0 + m) * R + q] = x[q];
     }
   }
@@ -495,7 +495,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
         need_lds -= 1;
         const T lv = a.val[ch.y];
         T cq = lv * inv;
-        cq = __builtin_fma(__builtin_fma(-cq, diag, lv), inv, cq);
+        cq = fma_t(fma_t(-cq, diag, lv), inv, cq);
         coef = -cq;
       }
 #pragma unroll
@@ -613,7 +613,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
           T r = ext ? s_val[k * R + q] : T(0);
           if constexpr (DIRECT) r = ext ? r - rem : T(0);
           T qv = r * inv;
-          qv = __builtin_fma(__builtin_fma(-qv, diag, r), inv, qv);
+          qv = fma_t(fma_t(-qv, diag, r), inv, qv);
           av[q] = active ? qv : xs[q];
         }
         T cf = active ? coef : T(0);
@@ -646,7 +646,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
 #pragma unroll
             for (int q = 0; q < R; q++) {
               const T ap = __shfl(av[q], pp);
-              av[q] = __builtin_fma(cf, ap, av[q]);
+              av[q] = fma_t(cf, ap, av[q]);
             }
             const T cp = __shfl(cf, pp);
             cf = cf * cp;

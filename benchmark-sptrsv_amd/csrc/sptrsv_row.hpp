@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(1024) k_exec_row(RowArgs<T> a) {
 #pragma unroll
           for (int u = 0; u < kRowU; u++)
 #pragma unroll
-            for (int q = 0; q < RPL; q++) acc[q] = __builtin_fma(v[u], xv[u][q], acc[q]);
+            for (int q = 0; q < RPL; q++) acc[q] = fma_t(v[u], xv[u][q], acc[q]);
           chunk = false;
           j += NS * kRowU;
           fin = j >= hi;
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(1024) k_exec_row(RowArgs<T> a) {
                 if (qv[q]) {
                   const T rr = bv[q] - sum[q];
                   T xr = rr * inv;
-                  xr = __builtin_fma(__builtin_fma(-xr, d, rr), inv, xr);
+                  xr = fma_t(fma_t(-xr, d, rr), inv, xr);
                   T *px = &a.x[(size_t)r * rhs + tile * (RL * RPL) + q * RL + lr];
                   if (kBulk)
                     st_agent(px, xr);
