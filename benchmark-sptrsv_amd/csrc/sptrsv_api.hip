@@ -856,9 +856,10 @@ struct PhaseTimer {
 // Levels of the unit DAG (units = runs of consecutive solve positions given
 // by d_ufirst[nunits+1] and d_uid[n], or single positions when both are null)
 // into d_lev[nunits], by the device pull sweep of k_unit_levels over the
-// plan's CSR.  Leaves the stream synchronised.
+// plan's CSR.  runs: the units are chain runs (k_level_runs) and a unit's
+// level is that of its last row.  Leaves the stream synchronised.
 int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
-                int nunits, int *d_lev, hipStream_t s) {
+                int nunits, int *d_lev, hipStream_t s, bool runs = false) {
   if (nunits <= 0) return SPTRSV_OK;
   ST_TRY(ensure_csr(p, s));
   ST_TRY(fill_async<int>(d_lev, (size_t)nunits, -1, s));
@@ -870,6 +871,7 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
   a.lev = d_lev;
   a.n = p->n;
   a.nunits = nunits;
+  a.runs = runs ? 1 : 0;
   a.backward = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
   a.ticket = p->sync.as<int>();
   a.err = p->sync.as<int>() + kErrWord;
@@ -882,7 +884,10 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
   // another (a 700^2 grid took 372 ms that way, 512 uneven grids 4 s).
   // Rows: 4 lanes per row when rows are short (16 rows per claim), 16 when
   // they are long; batches (up to hundreds of entries): a wavefront each.
-  const int G = d_ufirst ? 64 : (p->avg_offdiag <= 8.0 ? 4 : 16);
+  // chain runs: by their entries per unit
+  const double per_unit = runs ? (double)p->nnz / std::max(nunits, 1) : 0.0;
+  const int G = runs ? (per_unit <= 9.0 ? 4 : per_unit <= 33.0 ? 16 : 64)
+                     : d_ufirst ? 64 : (p->avg_offdiag <= 8.0 ? 4 : 16);
   a.rounds = 1;
   const long long per_claim = (long long)a.rounds * (kWave / G);
   const long long claims = (nunits + per_claim - 1) / per_claim;
@@ -899,6 +904,50 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
   return SPTRSV_OK;
 }
 
+// Levels of the rows (solve positions) into d_lev[n].  Runs of a chain
+// (k_level_runs) are swept as one unit each when they shrink the unit count
+// by a quarter or more: the sweep advances one dependent unit per step, so
+// a bidiagonal chain of 10^6 rows took 0.81 s as 10^6 units.
+int row_levels(sptrsv_plan_s *p, int *d_lev, hipStream_t s) {
+  const int n = p->n;
+  DevBuf start, incl;
+  ST_TRY(start.ensure(sizeof(int) * (size_t)n, false));
+  ST_TRY(incl.ensure(sizeof(int) * ((size_t)n + 1), false));
+  k_level_runs<<<grid_for(n, 256), 256, 0, s>>>(
+      p->colPtr, p->rowIdx, p->inDegree.as<int>(), n,
+      p->subst == SPTRSV_SUBSTITUTION_BACKWARD, start.as<int>());
+  HIP_TRY(hipGetLastError());
+  // exclusive scan of the flags followed by a 0: incl[p + 1] = units started
+  // at positions <= p, incl[n] = the unit count
+  HIP_TRY(hipMemcpyAsync(incl.as<int>(), start.as<int>(), sizeof(int) * (size_t)n,
+                         hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipMemsetAsync(incl.as<int>() + n, 0, sizeof(int), s));
+  ST_TRY(scan_rec(incl.as<int>(), n + 1, s));
+  int nunits = n;
+  HIP_TRY(hipMemcpyAsync(&nunits, incl.as<int>() + n, sizeof(int),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (nunits <= 0 || nunits > n) return SPTRSV_ERR_INVALID_ARG;
+  if (4LL * nunits > 3LL * n)
+    return unit_levels(p, nullptr, nullptr, n, d_lev, s);
+  DevBuf uid, ufirst, ulev;
+  ST_TRY(uid.ensure(sizeof(int) * (size_t)n, false));
+  ST_TRY(ufirst.ensure(sizeof(int) * ((size_t)nunits + 1), false));
+  ST_TRY(ulev.ensure(sizeof(int) * (size_t)nunits, false));
+  k_level_units<<<grid_for(n, 256), 256, 0, s>>>(start.as<int>(), incl.as<int>() + 1,
+                                                n, uid.as<int>(), ufirst.as<int>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(ufirst.as<int>() + nunits, &n, sizeof(int),
+                         hipMemcpyHostToDevice, s));
+  ST_TRY(unit_levels(p, ufirst.as<int>(), uid.as<int>(), nunits, ulev.as<int>(), s,
+                     true));
+  k_level_expand<<<grid_for(n, 256), 256, 0, s>>>(ulev.as<int>(), uid.as<int>(),
+                                                  ufirst.as<int>(), n, d_lev);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return SPTRSV_OK;
+}
+
 // DAG depth and level widths of the factor (findlevel.h:71-147's nlevel,
 // parallelism min/avg/max, including its else-if when tracking the minimum),
 // and the level order of the solve positions (p->lvlOrder: positions sorted
@@ -912,7 +961,7 @@ int plan_levels(sptrsv_plan_s *p, hipStream_t s) {
   ST_TRY(lev.ensure(sizeof(int) * (size_t)n, false));
   int st = cnt.ensure(sizeof(int) * ((size_t)n + 1), false);
   if (st == SPTRSV_OK) st = p->lvlOrder.ensure(sizeof(int) * (size_t)n, false);
-  if (st == SPTRSV_OK) st = unit_levels(p, nullptr, nullptr, n, lev.as<int>(), s);
+  if (st == SPTRSV_OK) st = row_levels(p, lev.as<int>(), s);
   std::vector<int> widths;
   int *d_cnt = cnt.as<int>();
   if (st == SPTRSV_OK) {

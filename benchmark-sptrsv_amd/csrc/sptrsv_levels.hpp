@@ -34,6 +34,7 @@ struct UnitLevelArgs {
   const int *__restrict__ uid;     // [n] unit of each solve position, or null
   int *lev;                        // [nunits], -1 until set
   int n, nunits, backward;
+  int runs;                        // units are chain runs: level of the last row
   int rounds;                      // rounds of 64 / G units per claim
   int *ticket;                     // [2]: claims, finished workgroups
   int *err;
@@ -119,7 +120,9 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
 #pragma unroll
           for (int off = G / 2; off > 0; off >>= 1) mm = max(mm, __shfl_xor(mm, off));
           if (gdone) {
-            if (gl == 0) xchg_agent(&a.lev[u], mm);
+            // a run unit's level is its last row's
+            const int ul = a.runs && a.ufirst ? a.ufirst[u + 1] - f - 1 : 0;
+            if (gl == 0) xchg_agent(&a.lev[u], mm + ul);
             pub = true;
           }
         }
@@ -139,6 +142,55 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
       xchg_agent(a.ticket, 0);
       xchg_agent(a.ticket + 1, 0);
     }
+  }
+}
+
+// Runs of a chain, for the row-level sweep: position p continues the run of
+// p - 1 when the row at p reads exactly one other column, the one at p - 1,
+// and that column is read by no other row.  The rows inside a run are then
+// seen by no row outside it, and the run's rows have consecutive levels, so
+// the sweep treats a run as one unit (its level: the level of its last row,
+// len - 1 more than its first row's) and k_level_expand restores the rows'
+// levels.  start[p] = 1 where a unit starts.
+__global__ void k_level_runs(const int *__restrict__ colPtr,
+                             const int *__restrict__ rowIdx,
+                             const int *__restrict__ inDeg, int n, int backward,
+                             int *start) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += gridDim.x * blockDim.x) {
+    int v = 1;
+    if (p > 0) {
+      const int c = backward ? n - p : p - 1;     // column at position p - 1
+      const int want = backward ? n - 1 - p : p;  // row at position p
+      const int f = colPtr[c], l = colPtr[c + 1];
+      const int j = backward ? f : f + 1;         // its one off-diagonal entry
+      if (l - f == 2 && rowIdx[j] == want && inDeg[want] == 2) v = 0;
+    }
+    start[p] = v;
+  }
+}
+
+// Unit of each position and first position of each unit, from the
+// inclusive scan of the start flags (ufirst[nunits] = n set by the caller).
+__global__ void k_level_units(const int *__restrict__ start,
+                              const int *__restrict__ incl, int n, int *uid,
+                              int *ufirst) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += gridDim.x * blockDim.x) {
+    const int u = incl[p] - 1;
+    uid[p] = u;
+    if (start[p]) ufirst[u] = p;
+  }
+}
+
+// Row levels from unit levels: a unit's level is its last row's.
+__global__ void k_level_expand(const int *__restrict__ ulev,
+                               const int *__restrict__ uid,
+                               const int *__restrict__ ufirst, int n, int *lev) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += gridDim.x * blockDim.x) {
+    const int u = uid[p];
+    lev[p] = ulev[u] - (ufirst[u + 1] - 1 - p);
   }
 }
 

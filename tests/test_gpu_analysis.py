@@ -18,7 +18,7 @@ import pytest
 import sptrsv_amd as S
 import sptrsv_oracle as orc
 from sptrsv_amd import matrices as mx
-from gpu_util import assert_close, oracle_x, require_gpu
+from gpu_util import assert_close, oracle_x, plan_solve, require_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -26,6 +26,27 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(autouse=True)
 def _gpu():
     require_gpu()
+
+
+def _chains_with_links(n, brk, extra, reach, seed):
+    """Chains with random breaks (a fraction `brk` of the subdiagonal left
+    out) and, for a fraction `extra` of the rows, one more input from up to
+    `reach` columns back: runs of pure links broken wherever a column or a
+    row has another entry, so the level sweep mixes run units and single
+    rows, and the levels restart at the breaks."""
+    rng = np.random.default_rng(seed)
+    sub = np.flatnonzero(rng.random(n - 1) >= brk)
+    rows_x = np.flatnonzero(rng.random(n) < extra)
+    rows_x = rows_x[rows_x >= 2]
+    src = rows_x - rng.integers(2, reach + 1, size=len(rows_x))
+    keep = src >= 0
+    rows_x, src = rows_x[keep], src[keep]
+    cols = np.concatenate([np.arange(n), sub, src])
+    rows = np.concatenate([np.arange(n), sub + 1, rows_x])
+    vals = np.concatenate([np.full(n, 2.0), np.full(len(sub), -1.0),
+                           np.full(len(src), -0.5)])
+    cp, ri, vv = mx._from_columns(n, cols, rows, vals)
+    return mx.Factor(cp, ri, vv, name=f"chains{n}_{brk}_{extra}_{reach}")
 
 
 def _factors():
@@ -36,10 +57,13 @@ def _factors():
         ("powerlaw_40000", mx.power_law_lower(40_000, window=4000, seed=3), None),
         ("stencil27_12x10x8", mx.stencil27_lower(12, 10, 8), None),
         ("batch_20", mx.laplacian2d_batch(20, lo=8, hi=40, seed=4), None),
+        # chain runs swept as one unit each (runs of 40-50 and of 3-10)
+        ("chains_long_runs", _chains_with_links(30_000, 0.01, 0.02, 300, seed=1), None),
+        ("chains_short_runs", _chains_with_links(30_000, 0.05, 0.2, 40, seed=2), None),
     ]
 
 
-@pytest.mark.parametrize("idx", range(6))
+@pytest.mark.parametrize("idx", range(8))
 @pytest.mark.parametrize("subst", [0, 1])
 def test_device_levels_match_findlevel(idx, subst):
     name, f, closed = _factors()[idx]
@@ -55,6 +79,30 @@ def test_device_levels_match_findlevel(idx, subst):
     assert plan.info()["levels"] == got[0]
     # the host entry point agrees too
     assert S.levels(f.colPtr, f.rowIdx) == got
+
+
+@pytest.mark.parametrize("subst", [0, 1])
+def test_device_levels_full_size_chain(subst):
+    # the 10^6 bidiagonal chain is one run: its levels take one unit's sweep
+    # (0.8 s as 10^6 single-row units), and the row engine, which claims
+    # rows in level order, still solves it
+    f = mx.bidiagonal(1_000_000)
+    if subst:
+        f = mx.upper_from_lower(f)
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val, substitution=subst)
+    import time
+    t0 = time.time()
+    got = plan.levels()
+    assert time.time() - t0 < 0.5
+    assert got == (1_000_000, 1, 1, 1)
+    g = _chains_with_links(200_000, 0.01, 0.02, 300, seed=3)
+    plan2 = S.Plan.from_host(g.colPtr, g.rowIdx, g.val)
+    assert plan2.levels() == tuple(orc.findlevel(g.colPtr, g.rowIdx))
+    b = g.spmv(mx.x_uniform(g.n, seed=2))
+    x, _ = plan_solve(g, b, plan=plan2, cfg=dict(engine=S.ENGINE_ROW, lanes_per_column=4,
+                                                 rhs_lanes=1, rhs_per_lane=1, block=256,
+                                                 window=1))
+    assert_close(x, oracle_x(g, b), 1e-12)
 
 
 def test_device_levels_full_size_powerlaw():
