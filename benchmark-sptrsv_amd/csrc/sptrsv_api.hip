@@ -971,7 +971,7 @@ int plan_levels(sptrsv_plan_s *p, hipStream_t s) {
         fill_async<int>(d_max, 1, -1, s) != SPTRSV_OK)
       st = SPTRSV_ERR_HIP;
     if (st == SPTRSV_OK) {
-      k_level_histogram<<<grid_for(n, 256), 256, 0, s>>>(lev.as<int>(), n, d_cnt,
+      k_level_histogram<<<(n + kLvlChunk - 1) / kLvlChunk, 256, 0, s>>>(lev.as<int>(), n, d_cnt,
                                                           d_max);
       if (hipGetLastError() != hipSuccess ||
           hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
