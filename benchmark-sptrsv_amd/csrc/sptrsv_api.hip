@@ -108,6 +108,73 @@ struct DevBuf {
   }
 };
 
+// Device temporaries of the analysis, kept by the plan between uses: a
+// hipMalloc of a few MB costs 0.1-0.3 ms and hipFree synchronises the
+// device, a dozen times per configuration.  The work that uses a temporary
+// runs on the analysis stream, so a buffer handed back and taken again is
+// reused in stream order.
+struct TmpPool {
+  static constexpr int kMax = 8;  // free buffers kept
+  std::vector<DevBuf> free;
+  // the smallest free buffer that fits, else the largest one grown
+  int take(size_t bytes, DevBuf *out) {
+    int best = -1;
+    for (int i = 0; i < (int)free.size(); i++)
+      if (free[i].bytes >= bytes && (best < 0 || free[i].bytes < free[best].bytes))
+        best = i;
+    if (best < 0)
+      for (int i = 0; i < (int)free.size(); i++)
+        if (best < 0 || free[i].bytes > free[best].bytes) best = i;
+    DevBuf b;
+    if (best >= 0) {
+      b = free[best];
+      free.erase(free.begin() + best);
+    }
+    // sizes rounded up to a power of two, so that an (n + 1)-int buffer
+    // serves later n-long int2 or 64-bit requests too
+    size_t want = 1;
+    while (want < bytes) want <<= 1;
+    const int st = b.bytes >= bytes ? SPTRSV_OK : b.ensure(want, false);
+    *out = b;
+    return st;
+  }
+  void give(DevBuf *b) {
+    if (b->p) {
+      if ((int)free.size() < kMax)
+        free.push_back(*b);
+      else
+        b->release();
+    }
+    b->p = nullptr;
+    b->bytes = 0;
+  }
+  size_t held() const {
+    size_t t = 0;
+    for (const DevBuf &b : free) t += b.bytes;
+    return t;
+  }
+  void clear() {
+    for (DevBuf &b : free) b.release();
+    free.clear();
+  }
+};
+
+// A temporary from the pool, handed back when it goes out of scope.
+struct TmpBuf : DevBuf {
+  TmpPool *pool;
+  explicit TmpBuf(TmpPool &tp) : pool(&tp) {}
+  TmpBuf(const TmpBuf &) = delete;
+  TmpBuf &operator=(const TmpBuf &) = delete;
+  int get(size_t bytes, bool zero, hipStream_t s) {
+    if (p) pool->give(this);
+    ST_TRY(pool->take(bytes, this));
+    if (zero && bytes > 0) HIP_TRY(hipMemsetAsync(p, 0, bytes, s));
+    return SPTRSV_OK;
+  }
+  void release() { pool->give(this); }
+  ~TmpBuf() { pool->give(this); }
+};
+
 // ------------------------------------------------------------------ scan --
 int scan_rec(int *d, int len, hipStream_t s) {
   if (len <= 1) {
@@ -408,6 +475,7 @@ struct sptrsv_plan_s {
   const int *rowIdx = nullptr;
   const void *val = nullptr;
   DevBuf inDegree;
+  TmpPool tmp;   // analysis temporaries
   DevBuf sync;  // int[kSyncWords]: [0] ticket, [1] finished, [kErrWord] err
   DevBuf gsum, gcnt;
   DevBuf slots64, slots32;  // lane-engine slots, kept at the empty sentinel
@@ -910,9 +978,9 @@ int unit_levels(sptrsv_plan_s *p, const int *d_ufirst, const int *d_uid,
 // a bidiagonal chain of 10^6 rows took 0.81 s as 10^6 units.
 int row_levels(sptrsv_plan_s *p, int *d_lev, hipStream_t s) {
   const int n = p->n;
-  DevBuf start, incl;
-  ST_TRY(start.ensure(sizeof(int) * (size_t)n, false));
-  ST_TRY(incl.ensure(sizeof(int) * ((size_t)n + 1), false));
+  TmpBuf start(p->tmp), incl(p->tmp);
+  ST_TRY(start.get(sizeof(int) * (size_t)n, false, s));
+  ST_TRY(incl.get(sizeof(int) * ((size_t)n + 1), false, s));
   k_level_runs<<<grid_for(n, 256), 256, 0, s>>>(
       p->colPtr, p->rowIdx, p->inDegree.as<int>(), n,
       p->subst == SPTRSV_SUBSTITUTION_BACKWARD, start.as<int>());
@@ -930,10 +998,10 @@ int row_levels(sptrsv_plan_s *p, int *d_lev, hipStream_t s) {
   if (nunits <= 0 || nunits > n) return SPTRSV_ERR_INVALID_ARG;
   if (4LL * nunits > 3LL * n)
     return unit_levels(p, nullptr, nullptr, n, d_lev, s);
-  DevBuf uid, ufirst, ulev;
-  ST_TRY(uid.ensure(sizeof(int) * (size_t)n, false));
-  ST_TRY(ufirst.ensure(sizeof(int) * ((size_t)nunits + 1), false));
-  ST_TRY(ulev.ensure(sizeof(int) * (size_t)nunits, false));
+  TmpBuf uid(p->tmp), ufirst(p->tmp), ulev(p->tmp);
+  ST_TRY(uid.get(sizeof(int) * (size_t)n, false, s));
+  ST_TRY(ufirst.get(sizeof(int) * ((size_t)nunits + 1), false, s));
+  ST_TRY(ulev.get(sizeof(int) * (size_t)nunits, false, s));
   k_level_units<<<grid_for(n, 256), 256, 0, s>>>(start.as<int>(), incl.as<int>() + 1,
                                                 n, uid.as<int>(), ufirst.as<int>());
   HIP_TRY(hipGetLastError());
@@ -957,9 +1025,9 @@ int plan_levels(sptrsv_plan_s *p, hipStream_t s) {
   if (p->nlevel > 0 || p->n == 0) return SPTRSV_OK;
   const int n = p->n;
   const auto t0 = std::chrono::steady_clock::now();
-  DevBuf lev, cnt;
-  ST_TRY(lev.ensure(sizeof(int) * (size_t)n, false));
-  int st = cnt.ensure(sizeof(int) * ((size_t)n + 1), false);
+  TmpBuf lev(p->tmp), cnt(p->tmp);
+  ST_TRY(lev.get(sizeof(int) * (size_t)n, false, s));
+  int st = cnt.get(sizeof(int) * ((size_t)n + 1), false, s);
   if (st == SPTRSV_OK) st = p->lvlOrder.ensure(sizeof(int) * (size_t)n, false);
   if (st == SPTRSV_OK) st = row_levels(p, lev.as<int>(), s);
   std::vector<int> widths;
@@ -1036,24 +1104,27 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   const int n = p->n;
   PhaseTimer pt(s);
   std::vector<int> wp(ws->nwin + 1, 0);
-  std::vector<unsigned char> link(std::max(n, 1), 0);
+  // chain links, two bits per position (k_links)
+  std::vector<unsigned char> link_q((size_t)std::max(n, 1) / 4 + 1, 0);
+  auto link = [&](int q) -> int { return (link_q[q >> 2] >> (2 * (q & 3))) & 3; };
   HIP_TRY(hipMemcpyAsync(wp.data(), ws->winPtr.p, sizeof(int) * wp.size(),
                          hipMemcpyDeviceToHost, s));
   if (n > 0) {
-    DevBuf dl;
-    ST_TRY(dl.ensure((size_t)n, false));
+    TmpBuf dl(p->tmp);
+    const size_t nq = ((size_t)n + 3) / 4;
+    ST_TRY(dl.get(nq, false, s));
     const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
     if (p->vt == SPTRSV_F64)
-      k_links<double><<<grid_for(n, 256), 256, 0, s>>>(
+      k_links<double><<<grid_for((int)nq, 256), 256, 0, s>>>(
           p->colPtr, p->rowIdx, (const double *)p->val, p->inDegree.as<int>(),
           n, bwd, dl.as<unsigned char>());
     else
-      k_links<float><<<grid_for(n, 256), 256, 0, s>>>(
+      k_links<float><<<grid_for((int)nq, 256), 256, 0, s>>>(
           p->colPtr, p->rowIdx, (const float *)p->val, p->inDegree.as<int>(),
           n, bwd, dl.as<unsigned char>());
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
-      e = hipMemcpyAsync(link.data(), dl.p, (size_t)n, hipMemcpyDeviceToHost, s);
+      e = hipMemcpyAsync(link_q.data(), dl.p, nq, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dl.release();
     if (e != hipSuccess) return SPTRSV_ERR_HIP;
@@ -1069,13 +1140,13 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
     std::vector<int> &loc = wloc[w];
     if (whi <= wlo) return;
     long links = 0;
-    for (int q = wlo + 1; q < whi; q++) links += link[q] != 0;
+    for (int q = wlo + 1; q < whi; q++) links += link(q) != 0;
     const bool chained = 2 * links >= (long)(whi - wlo - 1) && whi - wlo > 1;
     // run[q - wlo]: length of the chain run (columns joined by links) that
     // starts at q
     std::vector<int> run(whi - wlo, 1);
     for (int q = whi - 2; q >= wlo; q--)
-      if (link[q + 1]) run[q - wlo] = run[q + 1 - wlo] + 1;
+      if (link(q + 1)) run[q - wlo] = run[q + 1 - wlo] + 1;
     int start = wlo;
     bool pure = true;  // every link inside the current batch is pure
     loc.push_back(0);
@@ -1083,19 +1154,19 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
       const int len = q - start;
       // a run of pure links grows past a wavefront into a wide batch of up
       // to kWave * kWideC columns
-      const bool wide_ok = pure && link[q] == 2 && len < kWave * kWideC;
+      const bool wide_ok = pure && link(q) == 2 && len < kWave * kWideC;
       // at a chain break (a grid row's start) short runs merge into one
       // batch, but a batch never reaches into a run of kMinChainBatch or
       // more columns: batches stay aligned with the rows after a window
       // that starts mid-row
-      const bool brk = chained && !link[q] &&
+      const bool brk = chained && !link(q) &&
                        (len >= kMinChainBatch || run[q - wlo] >= kMinChainBatch);
       if (q > wlo && ((len >= kWave && !wide_ok) || brk)) {
         start = q;
         pure = true;
         loc.push_back(q - wlo);
       } else if (q > start) {
-        pure = pure && link[q] == 2;
+        pure = pure && link(q) == 2;
       }
     }
   };
@@ -1162,8 +1233,8 @@ int build_batches(sptrsv_plan_s *p, WindowSet *ws, DevBuf *bfirst, DevBuf *bid,
   pt.mark("batches: host deal + upload");
   std::vector<int> lev(nb, 0);
   if (n > 0 && !local.empty()) {
-    DevBuf dlev;
-    ST_TRY(dlev.ensure(sizeof(int) * nb, false));
+    TmpBuf dlev(p->tmp);
+    ST_TRY(dlev.get(sizeof(int) * nb, false, s));
     int st = unit_levels(p, bfirst->as<int>(), bid->as<int>(), (int)local.size(),
                          dlev.as<int>(), s);
     if (st == SPTRSV_OK &&
@@ -1215,8 +1286,8 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (n > 0) {
-    DevBuf cursor;
-    ST_TRY(cursor.ensure(sizeof(int) * (size_t)n, true));
+    TmpBuf cursor(p->tmp);
+    ST_TRY(cursor.get(sizeof(int) * (size_t)n, true, s));
     k_lane_targets<<<grid_for(n, 256), 256, 0, s>>>(
         p->colPtr, p->rowIdx, n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
         ws->winPtr.as<int>(), ws->nwin,
@@ -1231,13 +1302,15 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
   // k_lane_targets, whose etgt values it overwrites for the selected entries)
   pt.mark("slot targets");
   ST_TRY(ws->chain.ensure(sizeof(int2) * (size_t)std::max(n, 1), false));
-  DevBuf bfirst, bid;
+  TmpBuf bfirst(p->tmp), bid(p->tmp);  // at most n batches
+  ST_TRY(bfirst.get(sizeof(int) * ((size_t)n + 2), false, s));
+  ST_TRY(bid.get(sizeof(int) * (size_t)std::max(n, 1), false, s));
   std::vector<int> boff_h, bord_h;
   ST_TRY(build_batches(p, ws, &bfirst, &bid, &boff_h, &bord_h, s));
   pt.t = std::chrono::steady_clock::now();
   if (n > 0) {
-    DevBuf key;
-    ST_TRY(key.ensure(sizeof(unsigned long long) * (size_t)n, true));
+    TmpBuf key(p->tmp);
+    ST_TRY(key.get(sizeof(unsigned long long) * (size_t)n, true, s));
     const int bwd = p->subst == SPTRSV_SUBSTITUTION_BACKWARD;
     if (p->vt == SPTRSV_F64)
       k_chain_key<double><<<grid_for(n, 256), 256, 0, s>>>(
@@ -1265,8 +1338,8 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
   // batch is about to be solved; by row order, rows of the next grid plane
   // at a window's end would wait behind every earlier row of the window
   {
-    DevBuf pos;
-    ST_TRY(pos.ensure(sizeof(int) * ((size_t)n + 1), false));
+    TmpBuf pos(p->tmp);
+    ST_TRY(pos.get(sizeof(int) * ((size_t)n + 1), false, s));
     if (n > 0) {
       k_remote_flags<<<grid_for(n, 256), 256, 0, s>>>(
           ws->need.as<int2>(), n, p->subst == SPTRSV_SUBSTITUTION_BACKWARD,
@@ -1288,9 +1361,9 @@ int get_lane_targets(sptrsv_plan_s *p, const sptrsv_config_t &c, WindowSet *ws,
     // shift[b]: where batch b's entries move (processing order) minus where
     // they start in solve order
     const int nbt = (int)bord_h.size();
-    DevBuf bpos, shift;
-    if (st == SPTRSV_OK) st = bpos.ensure(sizeof(int) * ((size_t)nbt + 1), false);
-    if (st == SPTRSV_OK) st = shift.ensure(sizeof(int) * (size_t)nbt, false);
+    TmpBuf bpos(p->tmp), shift(p->tmp);
+    if (st == SPTRSV_OK) st = bpos.get(sizeof(int) * ((size_t)nbt + 1), false, s);
+    if (st == SPTRSV_OK) st = shift.get(sizeof(int) * (size_t)nbt, false, s);
     if (st == SPTRSV_OK && n > 0 && boff_h.back() > 0) {
       k_batch_entry_pos<<<grid_for(nbt + 1, 256), 256, 0, s>>>(
           bfirst.as<int>(), pos.as<int>(), boff_h.back(), n, bpos.as<int>());
@@ -2422,6 +2495,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
                     &p->csrIdx,   &p->csrVal,  &p->xtmp,    &p->r32,
                     &p->d32,      &p->csrVal32, &p->farcnt, &p->lvlOrder, &p->lvlRows};
   for (DevBuf *b : bufs) b->release();
+  p->tmp.clear();
   delete p;
   return SPTRSV_OK;
 }

@@ -940,7 +940,8 @@ __global__ void k_lane_targets(const int *__restrict__ colPtr,
   }
 }
 
-// Chain links by solve position: link[p] = 1 when the column at position
+// Chain links by solve position (packed two bits per position, four per
+// byte): link(p) = 1 when the column at position
 // p - 1 has an entry in the row at position p (the nearest off-diagonal
 // entry: first after the diagonal forward, last before it backward); 2 when
 // that link is pure: the column at p - 1 has no other off-diagonal entry,
@@ -953,21 +954,28 @@ __global__ void k_links(const int *__restrict__ colPtr,
                         const T *__restrict__ val,
                         const int *__restrict__ inDeg, int n, int backward,
                         unsigned char *link) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += gridDim.x * blockDim.x) {
-    unsigned char v = 0;
-    if (p > 0) {
-      const int c = backward ? n - p : p - 1;       // column at position p - 1
-      const int want = backward ? n - 1 - p : p;    // row at position p
-      const int f = colPtr[c], l = colPtr[c + 1];
-      const int j = backward ? l - 2 : f + 1;       // the nearest entry
-      if (j >= f && j < l && (backward ? j < l - 1 : j > f) && rowIdx[j] == want) {
-        v = 1;
-        const T d = val[backward ? colPtr[want + 1] - 1 : colPtr[want]];
-        if (l - f == 2 && inDeg[want] == 2 && fabs(val[j]) <= fabs(d)) v = 2;
+  // four positions per byte, two bits each (the host reads 1/4 the bytes)
+  const int nq = (n + 3) / 4;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq;
+       i += gridDim.x * blockDim.x) {
+    unsigned packed = 0;
+    for (int k = 0; k < 4; k++) {
+      const int p = 4 * i + k;
+      unsigned v = 0;
+      if (p > 0 && p < n) {
+        const int c = backward ? n - p : p - 1;       // column at position p - 1
+        const int want = backward ? n - 1 - p : p;    // row at position p
+        const int f = colPtr[c], l = colPtr[c + 1];
+        const int j = backward ? l - 2 : f + 1;       // the nearest entry
+        if (j >= f && j < l && (backward ? j < l - 1 : j > f) && rowIdx[j] == want) {
+          v = 1;
+          const T d = val[backward ? colPtr[want + 1] - 1 : colPtr[want]];
+          if (l - f == 2 && inDeg[want] == 2 && fabs(val[j]) <= fabs(d)) v = 2;
+        }
       }
+      packed |= v << (2 * k);
     }
-    link[p] = v;
+    link[i] = (unsigned char)packed;
   }
 }
 
