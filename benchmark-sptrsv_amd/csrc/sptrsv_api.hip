@@ -2646,6 +2646,7 @@ int sptrsv_plan_info(sptrsv_plan_t p, sptrsv_plan_info_t *info) {
                          kv.second->rlist.bytes + kv.second->roff.bytes +
                          kv.second->chain.bytes + kv.second->bstart.bytes +
                          kv.second->boff.bytes);
+  bytes += (long long)p->tmp.held();  // analysis temporaries kept for reuse
   info->scratch_bytes = bytes;
   info->analysis_ms = p->analysis_ms;
   info->levels = p->nlevel;
