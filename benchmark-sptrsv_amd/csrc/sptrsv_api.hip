@@ -110,9 +110,10 @@ struct DevBuf {
 
 // Device temporaries of the analysis, kept by the plan between uses: a
 // hipMalloc of a few MB costs 0.1-0.3 ms and hipFree synchronises the
-// device, a dozen times per configuration.  The work that uses a temporary
-// runs on the analysis stream, so a buffer handed back and taken again is
-// reused in stream order.
+// device, a dozen times per configuration.  Every analysis step that uses a
+// temporary synchronises its stream before the host reads its results, and
+// so before the buffer is handed back: a buffer taken again, on any stream,
+// is no longer in use.
 struct TmpPool {
   static constexpr int kMax = 8;  // free buffers kept
   std::vector<DevBuf> free;
