@@ -1678,11 +1678,13 @@ int prepare_exec(sptrsv_plan_s *p, int rhs, bool f32solve, Launch *L,
   return st;
 }
 
+// ir: the refinement fusions for an FP32 solve (row and lane engines), or null
 template <typename T>
 int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
-                T *x, int rhs, hipStream_t s) {
+                T *x, int rhs, hipStream_t s, const IrFuse *ir = nullptr) {
   if (p->n == 0) return SPTRSV_OK;
   const sptrsv_config_t &c = L.cfg;
+  const IrFuse fuse = ir ? *ir : IrFuse{nullptr, nullptr, nullptr};
   if (c.engine == SPTRSV_ENGINE_ROW) {
     // x starts at the sentinel: each solution is its own arrival flag
     ST_TRY(fill_async<T>(x, (size_t)p->n * rhs, slot_empty_host<T>(), s));
@@ -1700,6 +1702,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     ra.ticket = p->sync.as<int>();
     ra.err = p->sync.as<int>() + kErrWord;
     ra.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
+    ra.ir = fuse;
     RowKernel<T> kern = pick_row<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, c);
     if (!kern) return SPTRSV_ERR_INVALID_ARG;
     kern<<<L.grid, c.block, 0, s>>>(ra);
@@ -1735,6 +1738,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     la.ticket = p->sync.as<int>();
     la.err = p->sync.as<int>() + kErrWord;
     la.wait_cap = p->wait_cap_us * p->clock_khz / 1000;
+    la.ir = fuse;
     LaneKernel<T> kern =
         pick_lane<T>(p->subst == SPTRSV_SUBSTITUTION_BACKWARD, L.R,
                      L.ws->has_wide, c.remote == SPTRSV_REMOTE_DIRECT, L.cc);
@@ -1743,6 +1747,7 @@ int launch_exec(sptrsv_plan_s *p, const Launch &L, const T *val, const T *b,
     HIP_TRY(hipGetLastError());
     return SPTRSV_OK;
   }
+  if (fuse.b64 || fuse.xout) return SPTRSV_ERR_UNSUPPORTED;  // group engine
   ExecArgs<T> a;
   a.colPtr = p->colPtr;
   a.rowIdx = p->rowIdx;
@@ -1800,9 +1805,22 @@ int launch_ir(sptrsv_plan_s *p, const Launch &L, const double *b, double *x,
   float *r32 = p->r32.as<float>();
   float *d32 = p->d32.as<float>();
   const float *v32 = p->val32.as<float>();
-  k_to_f32<<<grid_for(len, 256), 256, 0, s>>>(b, r32, len);
-  HIP_TRY(hipGetLastError());
-  ST_TRY(launch_exec<float>(p, L, v32, r32, d32, rhs, s));
+  // the lane engine and the row engine with lanes over right-hand sides take
+  // b in FP64 in the first solve (no conversion pass), and those row solves
+  // and one-tile lane solves write the final x beside the last correction
+  // (no update pass; lane solves of several tiles would write it strided):
+  // 128 rhs on the 128^3 factor 8.36 -> 7.47 ms, the 128^3 factor with one
+  // rhs 0.76 -> 0.73 ms; the others run both passes
+  const bool row_bulk = L.cfg.engine == SPTRSV_ENGINE_ROW && L.cfg.rhs_lanes > 1;
+  const bool fused = row_bulk || L.cfg.engine == SPTRSV_ENGINE_LANE;
+  const bool fused_upd = row_bulk || (L.cfg.engine == SPTRSV_ENGINE_LANE && L.ntiles == 1);
+  IrFuse f{fused ? b : nullptr, nullptr, nullptr};
+  if (!fused) {
+    k_to_f32<<<grid_for(len, 256), 256, 0, s>>>(b, r32, len);
+    HIP_TRY(hipGetLastError());
+  }
+  if (fused_upd && iters == 0) f.xout = x;  // x = 0 + d
+  ST_TRY(launch_exec<float>(p, L, v32, r32, d32, rhs, s, &f));
   // ping-pong between the caller's x and the plan's scratch so that the last
   // update lands in x
   double *bufs[2] = {x, p->xtmp.as<double>()};
@@ -1814,12 +1832,18 @@ int launch_ir(sptrsv_plan_s *p, const Launch &L, const double *b, double *x,
         p->csrPtr.as<int>(), p->csrIdx.as<int>(), p->csrVal.as<double>(),
         p->n, rhs, b, x_old, d32, x_new, r32);
     HIP_TRY(hipGetLastError());
-    ST_TRY(launch_exec<float>(p, L, v32, r32, d32, rhs, s));
+    // the last correction writes x = x_new + d itself (in place when x_new
+    // is x: each element is read and written by the lane that solves it)
+    f = IrFuse{nullptr, nullptr, nullptr};
+    if (fused_upd && it == iters) f = IrFuse{nullptr, x_new, x};
+    ST_TRY(launch_exec<float>(p, L, v32, r32, d32, rhs, s, &f));
     x_old = x_new;
     cur ^= 1;
   }
-  k_update<<<grid_for(len, 256), 256, 0, s>>>(x_old, d32, x, len);
-  HIP_TRY(hipGetLastError());
+  if (!fused_upd) {
+    k_update<<<grid_for(len, 256), 256, 0, s>>>(x_old, d32, x, len);
+    HIP_TRY(hipGetLastError());
+  }
   return SPTRSV_OK;
 }
 

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sptrsv {
 
 constexpr int kWave = 64;
@@ -65,6 +67,27 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) {
 __device__ __forceinline__ float fma_t(float a, float b, float c) {
   return __builtin_fmaf(a, b, c);
 }
+// Fusions for the FP32 solves of iterative refinement: the first solve reads
+// b in FP64 and rounds it as it loads it (no conversion pass over b), the
+// last stores x = xold + d in FP64 beside each d it solves (no update pass
+// over x).  Null pointers in every other solve; FP64 kernels ignore them.
+struct IrFuse {
+  const double *b64;   // first solve: the FP64 right-hand sides, or null
+  const double *xold;  // last solve: the iterate d corrects, or null (= 0)
+  double *xout;        // last solve: where xold + d goes, or null
+};
+template <typename T>
+__device__ __forceinline__ T load_b(const T *b, const IrFuse &f, size_t i) {
+  if constexpr (std::is_same<T, float>::value)
+    if (f.b64) return (float)f.b64[i];
+  return b[i];
+}
+template <typename T>
+__device__ __forceinline__ void ir_update(const IrFuse &f, size_t i, T d) {
+  if constexpr (std::is_same<T, float>::value)
+    if (f.xout) f.xout[i] = (f.xold ? f.xold[i] : 0.0) + (double)d;
+}
+
 template <typename T>
 __device__ __forceinline__ T ld_wg(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

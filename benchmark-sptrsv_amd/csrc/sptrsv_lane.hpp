@@ -83,6 +83,7 @@ struct LaneArgs {
   int *ticket;
   int *err;
   long long wait_cap;  // longest single wait, wall-clock ticks
+  IrFuse ir;           // refinement fusions (FP32 solves)
 };
 
 constexpr int kPF = 4;  // off-diagonal entries per column held in registers
@@ -470,7 +471,7 @@ __device__ __forceinline__ void lane_compute(const LaneArgs<T> &a, T *s_val,
       T bq[R];
 #pragma unroll
       for (int q = 0; q < R; q++)
-        bq[q] = !WIDE && q < rv ? a.b[(size_t)c * a.rhs + tile * R + q] : T(0);
+        bq[q] = !WIDE && q < rv ? load_b(a.b, a.ir, (size_t)c * a.rhs + tile * R + q) : T(0);
       int f, l, rp;
       int2 nd, ch;
       if (pf_bi == bi) { f = pf_f; l = pf_l; nd = pf_nd; ch = pf_ch; rp = pf_rp; }
@@ -879,7 +880,7 @@ __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) 
         const int q = R == 1 ? 0 : i - k * R;
         const int p = wlo + k;
         const int c = BWD ? n - 1 - p : p;
-        if (q < rv) v = a.b[(size_t)c * a.rhs + tile * R + q];
+        if (q < rv) v = load_b(a.b, a.ir, (size_t)c * a.rhs + tile * R + q);
       }
       s_val[i] = v;
     }
@@ -902,7 +903,11 @@ __global__ void __launch_bounds__(lane_max_block(R)) k_exec_lane(LaneArgs<T> a) 
       const int q = R == 1 ? 0 : i - k * R;
       const int p = wlo + k;
       const int c = BWD ? n - 1 - p : p;
-      if (q < rv) a.x[(size_t)c * a.rhs + tile * R + q] = s_val[i];
+      if (q < rv) {
+        const size_t e = (size_t)c * a.rhs + tile * R + q;
+        a.x[e] = s_val[i];
+        ir_update(a.ir, e, s_val[i]);
+      }
     }
   }
   if (threadIdx.x == 0) {

@@ -51,6 +51,7 @@ struct RowArgs {
   int *ticket;                     // [2]: claims, finished workgroups
   int *err;
   long long wait_cap;
+  IrFuse ir;                       // refinement fusions (FP32 solves)
 };
 
 constexpr int kRowU = 4;      // entries per lane per poll round
@@ -134,10 +135,18 @@ __global__ void __launch_bounds__(1024, row_min_waves(G, RL, RPL)) k_exec_row(Ro
       T d = T(1);
       T bv[RPL];
       bool qv[RPL];
+      // refinement fusions (FP32 solves with lanes over right-hand sides; a
+      // wavefront per row over its entries ran 11% slower on the power-law
+      // factor with them compiled in, used or not).  The last correction's
+      // iterate is loaded with b, so the FP64 update adds no wait to the
+      // publish.
+      constexpr bool kUpd = std::is_same<T, float>::value && RL > 1;
+      double xo[kUpd ? RPL : 1];
 #pragma unroll
       for (int q = 0; q < RPL; q++) {
         bv[q] = T(0);
         qv[q] = false;
+        if constexpr (kUpd) xo[q] = 0.0;
       }
       if (act) {
         const int idx = (int)(t / a.ntiles);
@@ -150,7 +159,14 @@ __global__ void __launch_bounds__(1024, row_min_waves(G, RL, RPL)) k_exec_row(Ro
         for (int q = 0; q < RPL; q++) {
           const int k = tile * (RL * RPL) + q * RL + lr;
           qv[q] = k < rhs;
-          if (ls == 0 && qv[q]) bv[q] = a.b[(size_t)r * rhs + k];
+          if (ls == 0 && qv[q]) {
+            if constexpr (kUpd) {
+              bv[q] = load_b(a.b, a.ir, (size_t)r * rhs + k);
+              if (a.ir.xout && a.ir.xold) xo[q] = a.ir.xold[(size_t)r * rhs + k];
+            } else {
+              bv[q] = a.b[(size_t)r * rhs + k];
+            }
+          }
         }
         j = (BWD ? s0 + 1 : s0) + ls;
         hi = BWD ? s1 : s1 - 1;
@@ -227,11 +243,14 @@ __global__ void __launch_bounds__(1024, row_min_waves(G, RL, RPL)) k_exec_row(Ro
                   const T rr = bv[q] - sum[q];
                   T xr = rr * inv;
                   xr = fma_t(fma_t(-xr, d, rr), inv, xr);
-                  T *px = &a.x[(size_t)r * rhs + tile * (RL * RPL) + q * RL + lr];
+                  const size_t e = (size_t)r * rhs + tile * (RL * RPL) + q * RL + lr;
+                  T *px = &a.x[e];
                   if (kBulk)
                     st_agent(px, xr);
                   else
                     xchg_agent(px, xr);
+                  if constexpr (kUpd)
+                    if (a.ir.xout) a.ir.xout[e] = xo[q] + (double)xr;
                 }
             }
             pub = true;
