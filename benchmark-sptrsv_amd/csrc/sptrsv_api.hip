@@ -36,7 +36,7 @@ constexpr long long kDefaultWaitCapUs = 30ll * 1000 * 1000;
 // cache line, the error flag (polled by every stall watch) in a line of its
 // own, so that polling it never contends with the ticket's atomics
 constexpr int kErrWord = 64;
-constexpr int kSyncWords = 128;
+constexpr int kSyncWords = 512;  // [kStripeBase, +32 * kStripes): striped tickets
 // an off-diagonal entry this many columns from the diagonal or more lies
 // outside the default lane window of any column
 constexpr int kFarDistance = 8192;

@@ -54,6 +54,31 @@ template <typename T>
 __device__ __forceinline__ T xchg_take(T *p, T empty) {
   return __hip_atomic_exchange(p, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Striped tickets: consecutive chunks of work are claimed from kStripes
+// counters, chunk c from counter c % S, so claim atomics go to S addresses
+// (on one address they serialise at ~11.5 ns each).  Claimant i (a
+// wavefront or a workgroup, S <= the number of claimants) always uses
+// counter i % S and gets its chunks in increasing order.  The lowest
+// unfinished chunk is then always claimed or about to be: every chunk of its
+// counter below it is finished, so that counter's claimants are free, and
+// one of them takes it next, provided every counter has a resident claimant
+// (workgroups are dispatched in order, so the first kStripes cover them).
+// Counter s is int kStripeBase + 32 * s of the
+// plan's sync words (its own 128-byte line); the last claimant to finish
+// resets them.
+constexpr int kStripes = 8;
+constexpr int kStripeBase = 128;
+__device__ __forceinline__ long long claim_chunk(int *sync, int stripe,
+                                                 int nstripes) {
+  const int k = atomicAdd(sync + kStripeBase + 32 * stripe, 1);
+  return (long long)k * nstripes + stripe;
+}
+__device__ __forceinline__ void reset_stripes(int *sync) {
+  for (int s = 0; s < kStripes; s++)
+    __hip_atomic_exchange(sync + kStripeBase + 32 * s, 0, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename T>
 __device__ __forceinline__ void add_agent(T *p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

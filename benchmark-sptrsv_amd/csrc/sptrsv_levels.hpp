@@ -15,8 +15,9 @@
 // level is 0 when no row in it reads a column of another unit, else one more
 // than the largest level among the units it reads.
 //
-// Units are claimed through a ticket in solve order, by wavefronts: a claim
-// is `rounds` rounds of 64 / G units (G lanes per unit).  A claimed unit reads
+// Units are claimed through a striped ticket in solve order, by wavefronts
+// (claim_chunk: eight counters, so the claim atomics do not queue on one
+// address): a claim is `rounds` rounds of 64 / G units (G lanes per unit).  A claimed unit reads
 // only lower units, claimed earlier by running wavefronts, and a wavefront
 // finishes its units in claim order, so the lowest unfinished unit always
 // has its inputs and the sweep completes with any grid size.
@@ -53,18 +54,23 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
   const unsigned long long gmask =
       (G == kWave ? ~0ull : ((1ull << (G % kWave)) - 1)) << (lane & ~(G - 1));
   const int n = a.n;
+  // a wavefront's claims come from its stripe of the ticket
+  const int wpb = blockDim.x / kWave;
+  const int nstripes = min(kStripes, (int)gridDim.x * wpb);
+  const int stripe = (int)((blockIdx.x * wpb + threadIdx.x / kWave) % nstripes);
   StallWatch watch;
   int level = 0;
   bool stopped = false;
   for (;;) {
     // no claim ahead: units in solve order form chains, and a unit held
     // while its wavefront works on earlier ones would stall its dependents
-    int claim = 0;
-    if (lane == 0) claim = stopped ? a.nunits : atomicAdd(a.ticket, chunk);
-    const int base = __shfl(claim, 0);
+    long long claim = 0;
+    if (lane == 0)
+      claim = stopped ? a.nunits : claim_chunk(a.ticket, stripe, nstripes) * chunk;
+    const long long base = __shfl(claim, 0);
     if (base >= a.nunits) break;
     for (int rd = 0; rd < a.rounds && !stopped; rd++) {
-      const int u = base + rd * kPer + grp;
+      const int u = (int)base + rd * kPer + grp;
       const bool act = u < a.nunits;
       int f = 0, j = 0, hi = 0;
       if (act) {
@@ -139,7 +145,7 @@ __global__ void __launch_bounds__(256) k_unit_levels(UnitLevelArgs a) {
   if (threadIdx.x == 0) {
     const int done = atomicAdd(a.ticket + 1, 1);
     if (done == (int)gridDim.x - 1) {
-      xchg_agent(a.ticket, 0);
+      reset_stripes(a.ticket);
       xchg_agent(a.ticket + 1, 0);
     }
   }

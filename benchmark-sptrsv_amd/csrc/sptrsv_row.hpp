@@ -17,7 +17,9 @@
 // entry order, then a fixed butterfly), so the result is deterministic.
 //
 // Work is claimed through a ticket, by workgroups, in the level order of the
-// rows (the analysis sorts the solve positions by DAG level): a claim is
+// rows (the analysis sorts the solve positions by DAG level; a ticket striped
+// over eight counters, as the level sweep claims, let the stripes drift apart
+// and made the power-law factor 14% slower): a claim is
 // `rounds` rounds of one item per group of the workgroup (item = (row,
 // right-hand-side tile), tiles fastest), consecutive items going to
 // different groups.  Items that follow each other in level order are mostly
