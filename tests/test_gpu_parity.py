@@ -412,3 +412,39 @@ def test_matrix_market_factor(tmp_path, backward):
     B = f.spmv(X)
     x, _ = plan_solve(f, B, subst=backward)
     assert_close(x, oracle_x(f, B, subst=backward), 1e-12)
+
+
+# The FP32 solves of refinement read b in FP64 themselves (lane engine, row
+# engine with lanes over right-hand sides) and the last one writes x = x_old
+# + d beside d (row engine with lanes over right-hand sides, one-tile lane
+# solves); the other layouts run the conversion and update passes.  Every
+# layout, with 0, 1 (the update in place: x_old is x itself) and 2 steps,
+# against the FP64 oracle.
+IR_LAYOUTS = [
+    ("lane-one-tile", 1, dict(engine=S.ENGINE_LANE, block=512)),
+    ("lane-three-tiles", 3, dict(engine=S.ENGINE_LANE, block=512, rhs_per_lane=1)),
+    ("row-rhs-lanes", 16, dict(engine=S.ENGINE_ROW, lanes_per_column=8, rhs_lanes=8,
+                              rhs_per_lane=2, block=1024)),
+    ("row-entry-lanes", 1, dict(engine=S.ENGINE_ROW, lanes_per_column=64, block=1024)),
+    ("group", 1, dict(engine=S.ENGINE_GROUP, lanes_per_column=64, block=256, window=256)),
+]
+
+
+@pytest.mark.parametrize("name,rhs,cfg", IR_LAYOUTS, ids=[t[0] for t in IR_LAYOUTS])
+def test_refinement_fused_and_pass_layouts(name, rhs, cfg):
+    f = mx.laplacian3d_lower(24, 20, 16)
+    X = mx.x_normal(f.n, rhs, seed=31) if rhs > 1 else mx.x_uniform(f.n, seed=31)
+    B = f.spmv(X)
+    plan = S.Plan.from_host(f.colPtr, f.rowIdx, f.val, flags=S.PLAN_VALIDATE | S.PLAN_MIXED)
+    plan.set_config(rhs, **cfg)
+    db = S.DeviceArray.from_host(B)
+    dx = S.DeviceArray(B.shape, np.float64)
+    x64 = oracle_x(f, B, rhs=rhs)
+    errs = []
+    for iters in (0, 1, 2):
+        dx.zero()
+        plan.solve_ir(db, dx, rhs=rhs, iters=iters)
+        assert plan.check() == 0
+        errs.append(rel_errs(dx.to_host(), x64)[0])
+    assert 1e-12 < errs[0] <= 1e-5, errs   # an FP32 solve, rounded b
+    assert errs[1] <= 1e-9 and errs[2] <= 1e-10, errs
