@@ -73,6 +73,17 @@ __device__ __forceinline__ float rmw_read_val(float *p) {
   return __builtin_bit_cast(float, v);
 }
 
+// Waves per SIMD the compiler must leave room for.  Lanes over right-hand
+// sides with two per lane: 8 (at most 64 VGPRs, two spilled, so that two
+// 1024-thread workgroups fit a CU; unbounded they took 70 VGPRs, one
+// workgroup per CU, and 128 rhs on the 128^3 factor 3.49 -> 3.07 ms).  The
+// others are left alone: 32 lanes per row over its entries take 65 VGPRs,
+// and bounded to 64 (two workgroups per CU) the 27-point stencil ran 12%
+// slower; one right-hand side per lane fits 64 already, four take ~110.
+constexpr int row_min_waves(int G, int RL, int RPL) {
+  return RL > 1 && RPL == 2 ? 8 : 1;
+}
+
 // G lanes per item: RL of them over the tile's right-hand sides (RPL values
 // each, tile width RL * RPL), NS = G / RL over the row's entries.  RL = 1 is
 // the per-(row, right-hand side) pull of a single solution; RL = G reads whole
@@ -89,17 +100,6 @@ __device__ __forceinline__ float rmw_read_val(float *p) {
 // the value unchanged and an exchange; with whole rows of x, agent-scope
 // atomic loads and stores (coalesced, and a right-hand-side tile of a row is
 // hundreds of values: RMWs would serialise on the atomic units).
-// Waves per SIMD the compiler must leave room for.  Lanes over right-hand
-// sides with two per lane: 8 (at most 64 VGPRs, two spilled, so that two
-// 1024-thread workgroups fit a CU; unbounded they took 70 VGPRs, one
-// workgroup per CU, and 128 rhs on the 128^3 factor 3.49 -> 3.07 ms).  The
-// others are left alone: 32 lanes per row over its entries take 65 VGPRs,
-// and bounded to 64 (two workgroups per CU) the 27-point stencil ran 12%
-// slower; one right-hand side per lane fits 64 already, four take ~110.
-constexpr int row_min_waves(int G, int RL, int RPL) {
-  return RL > 1 && RPL == 2 ? 8 : 1;
-}
-
 template <typename T, bool BWD, int G, int RL, int RPL>
 __global__ void __launch_bounds__(1024, row_min_waves(G, RL, RPL)) k_exec_row(RowArgs<T> a) {
   constexpr int NS = G / RL;  // lanes over entries
